@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Throughput of the k-mer counting -> histogram cutoff -> macro-node path on
+one MI355X.
+
+One step = pk_count + pk_cutoff + pk_macro_nodes over the whole read set of
+the workload, with the FASTA bytes already resident in HBM.  Prints one JSON
+line (metric, value in k-mer instances per second, roofline of the dominant
+kernel, CPU baseline of the oracle on a bounded sample).
+
+  python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload bacterial]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # B/s, MI355X
+METRIC = "k-mers/s + % HBM peak, batched/served vs plain; p99 job latency at load"
+
+WORKLOADS = {
+    "bacterial": ("config_bacterial", 32,
+                  "bacterial: 4.6 Mbp genome at 50% GC, 100 bp reads, 80x, 1% errors, k=32"),
+    "small": ("config_small", 32, "small: 1 Mbp genome, 100 bp reads, 30x, 0.5% errors, k=32"),
+    "large": ("config_large", 64,
+              "large with repeats: 100 Mbp genome (5% from 50 repeat families, 2% divergence), "
+              "150 bp reads, 50x, 0.5% errors, k=64"),
+}
+
+
+def alg_bytes(N, U, V, M, S, W):
+    """Algorithmic HBM bytes per kernel (each datum read or written once)."""
+    return {
+        "hist": S,                              # read the FASTA
+        "scatter": S + N * W,                   # read the FASTA, write the instances
+        "sort": N * W + U * (W + 4),            # read the instances, write (key, count)
+        "compact": 2 * U * (W + 4),             # read and write the unique table
+        "cutoff": U * 4 + U * (W + 4) + V * (W + 4),
+        "mn": 2 * V * (W + 4) + M * (W + 32),
+        # whole path, one read of the input and one write + read of the instance stream
+        "pipeline": (S + 2 * N * W + U * (W + 4) + U * 4 + U * (W + 4) + V * (W + 4)
+                     + 2 * V * (W + 4) + M * (W + 32)),
+    }
+
+
+def cpu_baseline(fa, k, t, frac):
+    """Oracle (C++ restatement of the reference's algorithm, 1 thread) on the
+    first `frac` of the reads: count + cutoff + macro nodes."""
+    import numpy as np
+    from oracle import oracle as O
+    O.build()
+    arr = np.asarray(fa)
+    nl = np.flatnonzero(arr == 10)
+    R = nl.size // 2
+    take = max(1, int(R * frac))
+    end = int(nl[2 * take - 1]) + 1
+    sample = arr[:end]
+    t0 = time.perf_counter()
+    keys, counts = O.count(sample, k)
+    hist, mb, keep, rc = O.cutoff(counts, t)
+    if rc == 0:
+        O.macro_nodes(keys[keep], counts[keep], k)
+    dt = time.perf_counter() - t0
+    n_inst = int(counts.astype(np.int64).sum())
+    return {"value": n_inst / dt, "unit": "k-mers/s", "cores": 1, "kind": "port",
+            "sample": f"first {take} of {R} reads ({frac:.0%}) of the same workload: "
+                      f"{n_inst} k-mer instances, count + cutoff + macro nodes, 1 thread, "
+                      f"{dt:.1f} s"}
+
+
+def load_traffic(kernel, workload):
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        d = json.load(open(p))
+        return d.get(f"{kernel}:{workload}")
+    except Exception:
+        return None
+
+
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A plain device copy of known size, for scaling PMC byte counters."""
+    import torch
+    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda:0")
+    b = torch.empty_like(a)
+    a.fill_(1)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": nbytes, "bytes_written": nbytes, "reps": reps}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="bacterial", choices=sorted(WORKLOADS))
+    ap.add_argument("--t", type=int, default=21)
+    ap.add_argument("--cpu-frac", type=float, default=0.25)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "exchange of this path is a design, not a build", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from pakman_plus_amd import api, gen
+
+    if args.copy_calibration:
+        copy_calibration()
+        return
+
+    fn, k, desc = WORKLOADS[args.workload]
+    spec = getattr(gen, fn)()
+    fa = gen.make_fasta(spec)
+    nbytes = int(fa.nbytes)
+    dev = torch.from_numpy(fa).to("cuda:0")
+    torch.cuda.synchronize()
+    ctx = api.Context(0)
+
+    def step():
+        ctx.count(None, k, device_ptr=dev.data_ptr(), nbytes=nbytes)
+        ctx.cutoff(args.t)
+        ctx.macro_nodes()
+        return ctx.timing()
+
+    for _ in range(args.warmup):
+        step()
+    keys = ("parse_ms", "hist_ms", "scatter_ms", "sort_ms", "split_ms", "compact_ms", "cutoff_ms", "mn_ms")
+    acc = {x: 0.0 for x in keys}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tm = step()
+        for x in keys:
+            acc[x] += tm[x]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    N = int(tm["n_instances"])
+    U = ctx.table.n
+    V = ctx.n_kept
+    M = ctx.n_nodes
+    W = 8 if k <= 32 else 16
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = N * args.steps / elapsed
+    avg = {x.replace("_ms", ""): acc[x] / args.steps for x in keys}
+    ab = alg_bytes(N, U, V, M, nbytes, W)
+    # dominant kernel: the longest per-launch time among the single-kernel stages
+    cand = {x: avg[x] for x in ("hist", "scatter", "sort", "compact") if avg[x] > 0}
+    dom = max(cand, key=cand.get)
+    achieved = ab[dom] / (avg[dom] * 1e-3)
+    traffic = load_traffic(dom, args.workload)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "k-mers/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64" if k <= 32 else "u128 (two u64 words)",
+        "data": "synthetic (seeded generator, pakman_plus_amd.gen)",
+        "config": {"workload": desc, "k": k, "t": args.t, "reads": int(tm["n_reads"]),
+                   "kmer_instances": N, "unique_kmers": U, "kept_kmers": V, "macro_nodes": M,
+                   "fasta_bytes": nbytes, "bucket_bits": int(tm["bucket_bits"]),
+                   "split_buckets": int(tm["n_split"]), "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "alg_bytes_per_launch": ab[dom], "avg_ms": avg[dom]},
+        "pipeline": {"alg_bytes": ab["pipeline"], "frac_of_peak": ab["pipeline"] / (ms_per_step * 1e-3) / HBM_PEAK,
+                     "stage_ms": avg},
+    }
+    del dev
+    ctx.close()
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(fa, k, args.t, args.cpu_frac)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

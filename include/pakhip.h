@@ -1,0 +1,137 @@
+/* pakhip.h -- C ABI of the MI355X (gfx950) k-mer counting, histogram cutoff
+ * and macro-node construction library (libpakhip.so).
+ *
+ * Plain C: pointers, sizes and integer codes only.  Every entry point
+ * returns PK_OK (0) or a negative PK_E* code; pk_last_error() gives the
+ * message of the last failure on a context.  Nothing throws across the ABI.
+ *
+ * Reference interfaces replaced (paths relative to the vkundeti/pakman_plus
+ * source tree):
+ *   pk_count            perform_kmer_counting(const char*, size_t)
+ *                       src/kmer_counting.cpp:3351 -> global kmer_proc_buf
+ *                       (src/distribute_kmers.cpp:87)
+ *   pk_cutoff           construct_kmer_histogram(...)  src/kmer_histogram.cpp:72
+ *   pk_macro_nodes      construct_macro_nodes(MN_map, kmers)  src/mn_construct.cpp:212
+ *                       (pk_cutoff + pk_macro_nodes = begin_mnode_construction,
+ *                       src/mn_construct.cpp:479)
+ *   pk_builder_count    Concurrent_Kmer_Partial_Counter<kmer_t>::
+ *                       default_count_and_process(processor)
+ *                       src/concurrent_kmer_counter.hpp:2018-2082
+ *   pk_batch_count      no single counterpart: many independent read sets,
+ *                       each with the pk_count semantics, in one call
+ *
+ * Threading: a context is used by one host thread at a time; contexts are
+ * independent (each owns a HIP stream and its device memory).
+ */
+#ifndef PAKHIP_H
+#define PAKHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PK_OK 0
+#define PK_EINVAL (-1)  /* bad argument (k outside [8, 64], t < 0, ...) */
+#define PK_ENOMEM (-2)  /* device allocation failed / over the context budget */
+#define PK_EHIP (-3)    /* a HIP runtime call failed */
+#define PK_EFORMAT (-4) /* input is not strict single-line FASTA where the
+                           reference asserts (src/kmer_counting.cpp:2928) */
+#define PK_EEMPTY (-5)  /* no k-mer survives the cutoff (the reference calls
+                           MPI_Abort, src/kmer_histogram.cpp:189-200) */
+#define PK_ESTATE (-6)  /* call out of order (e.g. cutoff before count) */
+
+typedef struct pk_ctx pk_ctx;
+
+/* Sorted (ascending unsigned key), unique k-mer table.  keys holds n*words
+ * uint64 (words = 1 for k <= 32, 2 for 32 < k <= 64, stored lo then hi);
+ * counts are int32 as in the reference's KmerPairs (distribute_kmers.h:702-714).
+ * Pointers are device pointers owned by the context, valid until the next
+ * call that rebuilds the table or pk_ctx_destroy. */
+typedef struct pk_table {
+  uint64_t n;
+  int32_t words;
+  int32_t k;
+  const uint64_t* keys;
+  const int32_t* counts;
+} pk_table;
+
+/* One macro node (a (k-1)-mer key with its 1-base extensions).  Index
+ * b = base code (A=0 C=1 T=2 G=3, distribute_kmers.h:784-786).
+ * pre_cnt[a] = count of k-mer a.key (prefix a), suf_cnt[b] = count of
+ * k-mer key.b (suffix b); 0 = absent.  Nodes are ascending by key. */
+typedef struct pk_mnode {
+  uint64_t key_lo;
+  uint64_t key_hi;
+  int32_t pre_cnt[4];
+  int32_t suf_cnt[4];
+} pk_mnode;
+
+typedef struct pk_mn_table {
+  uint64_t n;
+  int32_t k;
+  const pk_mnode* nodes; /* device pointer owned by the context */
+} pk_mn_table;
+
+/* Per-kernel timings of the last pk_count / pk_cutoff / pk_macro_nodes
+ * (milliseconds, HIP events recorded on the context stream right around
+ * the kernel launches of each stage). */
+typedef struct pk_timing {
+  float parse_ms;    /* line index + read table */
+  float hist_ms;     /* bucket histogram (parse pass 1) */
+  float scatter_ms;  /* bucket scatter (parse pass 2) */
+  float sort_ms;     /* per-bucket sort + run-length count */
+  float split_ms;    /* oversize-bucket splitting (0 when none) */
+  float compact_ms;  /* dense table assembly */
+  float cutoff_ms;   /* histogram, min_bucket, kept-table compaction */
+  float mn_ms;       /* macro-node construction */
+  uint64_t n_instances; /* k-mer instances of the last pk_count */
+  uint64_t n_reads;     /* reads with at least one byte of sequence */
+  int32_t bucket_bits;  /* partition width used by the last pk_count */
+  int32_t n_split;      /* buckets that needed splitting */
+} pk_timing;
+
+/* Create a context on HIP device `device`.  hbm_budget_bytes = 0 means no
+ * limit beyond the device's memory. */
+int pk_ctx_create(pk_ctx** out, int device, uint64_t hbm_budget_bytes);
+void pk_ctx_destroy(pk_ctx* ctx);
+const char* pk_last_error(const pk_ctx* ctx);
+/* The context's hipStream_t (as void*). */
+void* pk_ctx_stream(pk_ctx* ctx);
+/* Device bytes currently held by the context. */
+uint64_t pk_ctx_device_bytes(const pk_ctx* ctx);
+
+/* Count the k-mers of a strict single-line FASTA buffer (main path of the
+ * reference, kmer_counting.cpp:2890-3261).  fasta may be a host pointer
+ * (copied in) or, with fasta_on_device != 0, a device pointer that stays
+ * resident.  8 <= k <= 64.  On success *out describes the table. */
+int pk_count(pk_ctx* ctx, const char* fasta, uint64_t nbytes, int k, int fasta_on_device,
+             pk_table* out);
+
+/* Histogram cutoff on the current table (kmer_histogram.cpp:84-140):
+ * hist_t[c] (host array of t entries) = number of k-mers with count c < t;
+ * *min_bucket = first argmin of hist_t[1..t-1] (t == 1 -> 1); the kept
+ * table = entries with count >= *min_bucket, in table order.  t == 0 uses
+ * the reference's default 21 (distribute_kmers.cpp:291-294).  Returns
+ * PK_EEMPTY when nothing is kept. */
+int pk_cutoff(pk_ctx* ctx, int t, uint64_t* hist_t, int* min_bucket, uint64_t* n_kept);
+/* The kept table of the last pk_cutoff. */
+int pk_kept_table(pk_ctx* ctx, pk_table* out);
+
+/* Macro nodes of the kept table (mn_construct.cpp:212-477, one rank). */
+int pk_macro_nodes(pk_ctx* ctx, pk_mn_table* out);
+
+/* Copy the current table / kept table / macro nodes to host memory.
+ * Arrays must hold n*words keys, n counts, n nodes. */
+int pk_download_table(pk_ctx* ctx, uint64_t* keys, int32_t* counts);
+int pk_download_kept(pk_ctx* ctx, uint64_t* keys, int32_t* counts);
+int pk_download_nodes(pk_ctx* ctx, pk_mnode* nodes);
+
+int pk_last_timing(const pk_ctx* ctx, pk_timing* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PAKHIP_H */

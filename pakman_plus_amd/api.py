@@ -1,0 +1,260 @@
+"""Host-side mirror of the reference's counting / cutoff / macro-node interface
+over the C ABI of libpakhip.so (include/pakhip.h).
+
+Reference entry points mirrored (src/ paths of vkundeti/pakman_plus):
+  perform_kmer_counting(read_data, length)      kmer_counting.cpp:3351
+  construct_kmer_histogram(...)                 kmer_histogram.cpp:72
+  construct_macro_nodes(MN_map, kmers)          mn_construct.cpp:212
+  begin_mnode_construction(MN_map)              mn_construct.cpp:479
+Errors are raised as PakHipError carrying the PK_* code; the reference's
+aborts map to PK_EFORMAT (bad FASTA) and PK_EEMPTY (nothing kept).
+
+The HIP library is the only implementation: if libpakhip.so is missing or no
+GPU is present, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PK_OK, PK_EINVAL, PK_ENOMEM, PK_EHIP, PK_EFORMAT, PK_EEMPTY, PK_ESTATE = 0, -1, -2, -3, -4, -5, -6
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libpakhip.so")
+
+
+class PakHipError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"pakhip error {code}: {msg}")
+        self.code = code
+
+
+class _Table(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("words", ctypes.c_int32), ("k", ctypes.c_int32),
+                ("keys", ctypes.c_void_p), ("counts", ctypes.c_void_p)]
+
+
+class _MnTable(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("k", ctypes.c_int32), ("nodes", ctypes.c_void_p)]
+
+
+class _Timing(ctypes.Structure):
+    _fields_ = [("parse_ms", ctypes.c_float), ("hist_ms", ctypes.c_float),
+                ("scatter_ms", ctypes.c_float), ("sort_ms", ctypes.c_float),
+                ("split_ms", ctypes.c_float), ("compact_ms", ctypes.c_float),
+                ("cutoff_ms", ctypes.c_float), ("mn_ms", ctypes.c_float),
+                ("n_instances", ctypes.c_uint64), ("n_reads", ctypes.c_uint64),
+                ("bucket_bits", ctypes.c_int32), ("n_split", ctypes.c_int32)]
+
+
+MNODE_DTYPE = np.dtype([("key_lo", "<u8"), ("key_hi", "<u8"), ("pre_cnt", "<i4", (4,)),
+                        ("suf_cnt", "<i4", (4,))])
+
+# every symbol declared in include/pakhip.h
+EXPORTS = ("pk_ctx_create", "pk_ctx_destroy", "pk_last_error", "pk_ctx_stream",
+           "pk_ctx_device_bytes", "pk_count", "pk_cutoff", "pk_kept_table", "pk_macro_nodes",
+           "pk_download_table", "pk_download_kept", "pk_download_nodes", "pk_last_timing")
+
+_LIB = None
+
+
+def load_library():
+    """Load libpakhip.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    # torch ships its own HIP runtime under the same soname; when torch is
+    # present it is loaded first so the process holds a single HIP runtime
+    # that both torch and libpakhip bind to.
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} not found: build it with `python -m pakman_plus_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, u64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64
+    L.pk_ctx_create.argtypes = [ctypes.POINTER(vp), i32, u64]
+    L.pk_ctx_create.restype = i32
+    L.pk_ctx_destroy.argtypes = [vp]
+    L.pk_ctx_destroy.restype = None
+    L.pk_last_error.argtypes = [vp]
+    L.pk_last_error.restype = ctypes.c_char_p
+    L.pk_ctx_stream.argtypes = [vp]
+    L.pk_ctx_stream.restype = vp
+    L.pk_ctx_device_bytes.argtypes = [vp]
+    L.pk_ctx_device_bytes.restype = u64
+    L.pk_count.argtypes = [vp, vp, u64, i32, i32, ctypes.POINTER(_Table)]
+    L.pk_count.restype = i32
+    L.pk_cutoff.argtypes = [vp, i32, vp, ctypes.POINTER(i32), ctypes.POINTER(u64)]
+    L.pk_cutoff.restype = i32
+    L.pk_kept_table.argtypes = [vp, ctypes.POINTER(_Table)]
+    L.pk_kept_table.restype = i32
+    L.pk_macro_nodes.argtypes = [vp, ctypes.POINTER(_MnTable)]
+    L.pk_macro_nodes.restype = i32
+    L.pk_download_table.argtypes = [vp, vp, vp]
+    L.pk_download_table.restype = i32
+    L.pk_download_kept.argtypes = [vp, vp, vp]
+    L.pk_download_kept.restype = i32
+    L.pk_download_nodes.argtypes = [vp, vp]
+    L.pk_download_nodes.restype = i32
+    L.pk_last_timing.argtypes = [vp, ctypes.POINTER(_Timing)]
+    L.pk_last_timing.restype = i32
+    _LIB = L
+    return L
+
+
+@dataclass
+class TableView:
+    """Device-resident sorted (k-mer, count) table (pointers owned by the Context)."""
+    n: int
+    words: int
+    k: int
+    keys_ptr: int
+    counts_ptr: int
+
+
+class Context:
+    """One device context (HIP stream + device memory) of libpakhip."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self._lib.pk_ctx_create(ctypes.byref(h), device, hbm_budget_bytes)
+        if rc:
+            raise PakHipError(rc, "pk_ctx_create failed (no usable HIP device?)")
+        self._h = h
+        self.k = None
+        self.table: TableView | None = None
+        self.n_kept = 0
+        self.n_nodes = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pk_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc:
+            raise PakHipError(rc, self._lib.pk_last_error(self._h).decode(errors="replace"))
+
+    @property
+    def stream(self) -> int:
+        return self._lib.pk_ctx_stream(self._h)
+
+    @property
+    def device_bytes(self) -> int:
+        return int(self._lib.pk_ctx_device_bytes(self._h))
+
+    # -- perform_kmer_counting ------------------------------------------------
+    def count(self, fasta, k: int, device_ptr: int | None = None, nbytes: int | None = None) -> TableView:
+        """Count k-mers.  `fasta` is bytes / a uint8 numpy array (copied to the
+        device), or pass device_ptr + nbytes of a device-resident buffer."""
+        t = _Table()
+        if device_ptr is not None:
+            rc = self._lib.pk_count(self._h, ctypes.c_void_p(device_ptr), nbytes, k, 1, ctypes.byref(t))
+        else:
+            a = np.frombuffer(fasta, dtype=np.uint8) if isinstance(fasta, (bytes, bytearray)) else np.ascontiguousarray(fasta, dtype=np.uint8)
+            rc = self._lib.pk_count(self._h, a.ctypes.data, a.nbytes, k, 0, ctypes.byref(t))
+        self._check(rc)
+        self.k = k
+        self.table = TableView(int(t.n), int(t.words), int(t.k), t.keys or 0, t.counts or 0)
+        return self.table
+
+    def table_host(self):
+        """(keys[n, words] uint64, counts[n] int32) of the current table."""
+        n, w = self.table.n, self.table.words
+        keys = np.empty((n, w), dtype=np.uint64)
+        counts = np.empty(n, dtype=np.int32)
+        self._check(self._lib.pk_download_table(self._h, keys.ctypes.data, counts.ctypes.data))
+        return keys, counts
+
+    # -- construct_kmer_histogram ---------------------------------------------
+    def cutoff(self, t: int = 21):
+        """(hist[t], min_bucket, n_kept).  Raises PakHipError(PK_EEMPTY) when
+        nothing survives (the reference aborts there)."""
+        tt = 21 if t == 0 else t
+        hist = np.zeros(max(tt, 1), dtype=np.uint64)
+        mb = ctypes.c_int()
+        nk = ctypes.c_uint64()
+        rc = self._lib.pk_cutoff(self._h, t, hist.ctypes.data, ctypes.byref(mb), ctypes.byref(nk))
+        self.n_kept = int(nk.value)
+        self.min_bucket = int(mb.value)
+        self.hist = hist
+        self._check(rc)
+        return hist, int(mb.value), int(nk.value)
+
+    def kept_host(self):
+        w = 1 if self.k <= 32 else 2
+        keys = np.empty((self.n_kept, w), dtype=np.uint64)
+        counts = np.empty(self.n_kept, dtype=np.int32)
+        self._check(self._lib.pk_download_kept(self._h, keys.ctypes.data, counts.ctypes.data))
+        return keys, counts
+
+    # -- construct_macro_nodes --------------------------------------------------
+    def macro_nodes(self) -> int:
+        m = _MnTable()
+        self._check(self._lib.pk_macro_nodes(self._h, ctypes.byref(m)))
+        self.n_nodes = int(m.n)
+        self.nodes_ptr = m.nodes or 0
+        return self.n_nodes
+
+    def nodes_host(self) -> np.ndarray:
+        out = np.empty(self.n_nodes, dtype=MNODE_DTYPE)
+        if self.n_nodes:
+            self._check(self._lib.pk_download_nodes(self._h, out.ctypes.data))
+        return out
+
+    def timing(self) -> dict:
+        t = _Timing()
+        self._check(self._lib.pk_last_timing(self._h, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in _Timing._fields_}
+
+
+def nodes_as_arrays(nodes: np.ndarray, k: int) -> dict:
+    """Node records as the arrays the oracle returns (keys[m, words], pre_cnt, suf_cnt)."""
+    w = 1 if k <= 32 else 2
+    keys = np.empty((nodes.shape[0], w), dtype=np.uint64)
+    keys[:, 0] = nodes["key_lo"]
+    if w == 2:
+        keys[:, 1] = nodes["key_hi"]
+    return {"keys": keys, "pre_cnt": np.ascontiguousarray(nodes["pre_cnt"]),
+            "suf_cnt": np.ascontiguousarray(nodes["suf_cnt"])}
+
+
+# ---------------------------------------------------------------------------
+# Function-style mirror of the reference's free functions.
+
+def perform_kmer_counting(ctx: Context, read_data, k: int = 32):
+    """kmer_counting.cpp:3351: returns the sorted unique (keys, counts) table
+    (the reference's kmer_proc_buf)."""
+    ctx.count(read_data, k)
+    return ctx.table_host()
+
+
+def construct_kmer_histogram(ctx: Context, num_buckets: int = 21):
+    """kmer_histogram.cpp:72: (hist, min_bucket, kept keys, kept counts)."""
+    hist, mb, _ = ctx.cutoff(num_buckets)
+    keys, counts = ctx.kept_host()
+    return hist, mb, keys, counts
+
+
+def begin_mnode_construction(ctx: Context, num_buckets: int = 21):
+    """mn_construct.cpp:479: cutoff + macro nodes; returns the node records."""
+    ctx.cutoff(num_buckets)
+    ctx.macro_nodes()
+    return ctx.nodes_host()

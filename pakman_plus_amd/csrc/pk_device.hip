@@ -1,0 +1,1745 @@
+// pk_device.hip -- MI355X (gfx950) k-mer counting, histogram cutoff and
+// macro-node construction behind the C ABI of include/pakhip.h.
+//
+// Pipeline of pk_count (one HIP stream per context, everything resident in HBM):
+//   1. line index     k_nl_count / scan / k_nl_write: positions of every '\n'
+//                     (16 B per thread, block-scan compaction).
+//   2. read table     k_reads: record r = header line j0+2r, sequence line
+//                     j0+2r+1, with the record rules and the format check of
+//                     the reference's Sliding_window (src/kmer_counting.cpp:
+//                     2890-2945); k-mer offsets by exclusive scan.
+//   3. partition      k_bucket_hist + k_bucket_scatter: every read is parsed
+//                     by one thread with the reference's rolling 3-bit update
+//                     (kmer_shift, distribute_kmers.h:798-805); the k-mer goes
+//                     to bucket = its top b1 bits.  Per-block LDS histograms,
+//                     one global atomic per (block, bucket) to reserve space.
+//                     This is the on-device stand-in for the reference's
+//                     owner routing + MPI_Alltoallv (kmer_counting.cpp:254-352).
+//   4. bucket count   k_bucket_sort: one 1024-thread workgroup per bucket holds
+//                     the bucket's keys in registers, counting-sorts them into
+//                     LDS by the next 12 bits in rounds of at most RC keys,
+//                     finishes each sub-bin (insertion sort, or a bitonic sort
+//                     of the round when a sub-bin is large) and run-length
+//                     encodes (the reference's sort + RLE + merge + aggregate,
+//                     kmer_counting.cpp:3057-3086, 169-250).  Buckets larger
+//                     than the register capacity are split further by
+//                     k_part_hist / k_part_scatter (host-driven levels).
+//   5. compaction     k_compact: bucket results -> dense sorted table.
+// pk_cutoff: k_count_hist (LDS histogram of counts < t), k_min_bucket, ordered
+//   stream compaction (kmer_histogram.cpp:84-140).
+// pk_macro_nodes: range-partitioned 5-way merge of the kept table seen as
+//   pred keys (ascending) and, per first base a, succ keys (ascending within
+//   the quarter) -- no sort is needed (mn_construct.cpp:212-477).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pakhip.h"
+
+typedef unsigned __int128 u128;
+typedef unsigned long long ull;
+
+// ---------------------------------------------------------------------------
+// Key helpers (Key = uint64_t for k <= 32, u128 for 32 < k <= 64).
+
+template <class Key>
+__host__ __device__ __forceinline__ Key key_mask(int nbits) {
+  const int bits = (int)sizeof(Key) * 8;
+  if (nbits >= bits) return ~Key(0);
+  return (Key(1) << nbits) - Key(1);
+}
+
+// bits [pos, pos+nbits) of x; bits below 0 read as 0.  nbits <= 16.
+template <class Key>
+__device__ __forceinline__ uint32_t key_digit(Key x, int pos, int nbits) {
+  if (nbits <= 0) return 0;
+  Key y;
+  if (pos >= 0) {
+    if (pos >= (int)sizeof(Key) * 8) return 0;
+    y = x >> pos;
+  } else {
+    y = x << (-pos);
+  }
+  return (uint32_t)y & ((1u << nbits) - 1u);
+}
+
+template <class Key>
+__device__ __forceinline__ Key load_key(const Key* p) { return *p; }
+
+// ---------------------------------------------------------------------------
+// Block-wide primitives (wave64).
+
+// Exclusive prefix sum of v over the block (all NT threads must call it).
+template <int NT>
+__device__ __forceinline__ ull block_excl_scan(ull v) {
+  constexpr int NW = NT / 64;
+  __shared__ ull wsum[NW];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  ull x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    ull y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    const ull s = lane < NW ? wsum[lane] : 0ull;
+    ull inc = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      ull y = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += y;
+    }
+    if (lane < NW) wsum[lane] = inc - s;
+  }
+  __syncthreads();
+  const ull r = x - v + wsum[w];
+  __syncthreads();
+  return r;
+}
+
+template <int NT>
+__device__ __forceinline__ ull block_sum(ull v) {
+  constexpr int NW = NT / 64;
+  __shared__ ull ws[NW];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (lane == 0) ws[w] = v;
+  __syncthreads();
+  ull s = 0;
+#pragma unroll
+  for (int i = 0; i < NW; i++) s += ws[i];
+  __syncthreads();
+  return s;
+}
+
+template <int NT>
+__device__ __forceinline__ ull block_max(ull v) {
+  constexpr int NW = NT / 64;
+  __shared__ ull ws[NW];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    ull y = __shfl_xor(v, o, 64);
+    v = y > v ? y : v;
+  }
+  if (lane == 0) ws[w] = v;
+  __syncthreads();
+  ull s = 0;
+#pragma unroll
+  for (int i = 0; i < NW; i++) s = ws[i] > s ? ws[i] : s;
+  __syncthreads();
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Small utility kernels.
+
+__global__ void k_fill_u64(ull* p, ull v, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_copy_u64(ull* dst, const ull* src) { *dst = *src; }
+
+// ---------------------------------------------------------------------------
+// Exclusive scan (u32 or u64 in, u64 out, out[n] = total).  4096 elements
+// per 1024-thread block, recursive over block sums.
+
+constexpr int SCAN_NT = 1024;
+constexpr int SCAN_PER = 4096;
+
+template <class T>
+__global__ __launch_bounds__(SCAN_NT) void k_scan_reduce(const T* in, ull* sums, uint64_t n) {
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER + (uint64_t)threadIdx.x * 4;
+  ull s = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    if (base + i < n) s += (ull)in[base + i];
+  s = block_sum<SCAN_NT>(s);
+  if (threadIdx.x == 0) sums[blockIdx.x] = s;
+}
+
+template <class T>
+__global__ __launch_bounds__(SCAN_NT) void k_scan_apply(const T* in, ull* out, const ull* offs,
+                                                        uint64_t n) {
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER + (uint64_t)threadIdx.x * 4;
+  ull v[4];
+  ull s = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    v[i] = (base + i < n) ? (ull)in[base + i] : 0ull;
+    s += v[i];
+  }
+  ull ex = block_excl_scan<SCAN_NT>(s);
+  ull o = (offs ? offs[blockIdx.x] : 0ull) + ex;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    if (base + i < n) out[base + i] = o;
+    o += v[i];
+  }
+  // the thread owning element n-1 writes the total
+  if (n > 0 && base <= n - 1 && n - 1 < base + 4) out[n] = o;
+}
+
+// ---------------------------------------------------------------------------
+// Line index.
+
+constexpr int NL_NT = 256;
+constexpr int NL_PER_THREAD = 16;
+constexpr uint64_t NL_PER_BLOCK = (uint64_t)NL_NT * NL_PER_THREAD;
+
+__device__ __forceinline__ void load16(const uint8_t* fa, uint64_t n, uint64_t base, uint8_t* b) {
+  if (base + 16 <= n && ((reinterpret_cast<uintptr_t>(fa + base) & 15) == 0)) {
+    uint4 v = *reinterpret_cast<const uint4*>(fa + base);
+    memcpy(b, &v, 16);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 16; i++) b[i] = (base + i < n) ? fa[base + i] : 0;
+  }
+}
+
+__global__ __launch_bounds__(NL_NT) void k_nl_count(const uint8_t* fa, uint64_t n, ull* blk_cnt,
+                                                    ull* first_gt) {
+  const uint64_t base = (uint64_t)blockIdx.x * NL_PER_BLOCK + (uint64_t)threadIdx.x * NL_PER_THREAD;
+  uint8_t b[16];
+  load16(fa, n, base, b);
+  ull c = 0, gt = ~0ull;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    if (base + i < n) {
+      c += (b[i] == '\n');
+      if (b[i] == '>' && gt == ~0ull) gt = base + i;
+    }
+  }
+  // wave min of the first '>' then one atomic per wave
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    ull y = __shfl_xor(gt, o, 64);
+    gt = y < gt ? y : gt;
+  }
+  if ((threadIdx.x & 63) == 0 && gt != ~0ull) atomicMin(first_gt, gt);
+  c = block_sum<NL_NT>(c);
+  if (threadIdx.x == 0) blk_cnt[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(NL_NT) void k_nl_write(const uint8_t* fa, uint64_t n, const ull* blk_off,
+                                                    ull* nl) {
+  const uint64_t base = (uint64_t)blockIdx.x * NL_PER_BLOCK + (uint64_t)threadIdx.x * NL_PER_THREAD;
+  uint8_t b[16];
+  load16(fa, n, base, b);
+  ull c = 0;
+#pragma unroll
+  for (int i = 0; i < 16; i++) c += (base + i < n && b[i] == '\n');
+  ull o = block_excl_scan<NL_NT>(c) + blk_off[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < 16; i++)
+    if (base + i < n && b[i] == '\n') nl[o++] = base + i;
+}
+
+// Records (kmer_counting.cpp:2920-2945).  Candidate r uses header newline
+// nl[j0+2r] and sequence newline nl[j0+2r+1] (or the end of the buffer).
+__global__ void k_reads(const uint8_t* fa, uint64_t n, const ull* nl, uint64_t m, uint64_t p0,
+                        int K, uint64_t ncand, ull* rstart, uint32_t* rlen, uint32_t* kc,
+                        ull* first_unproc, ull* first_err, ull* too_long) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ncand) return;
+  // j0 = first newline at or after p0
+  uint64_t lo = 0, hi = m;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (nl[mid] < p0) lo = mid + 1; else hi = mid;
+  }
+  const uint64_t j0 = lo;
+  const uint64_t h = j0 + 2 * r;
+  bool processed = false;
+  if (p0 < n && h < m) {
+    const uint64_t hdr_nl = nl[h];
+    processed = hdr_nl + 1 + (uint64_t)K <= n;
+    if (processed) {
+      const uint64_t s = hdr_nl + 1;
+      const uint64_t e = (h + 1 < m) ? nl[h + 1] : n;
+      const uint64_t len = e - s;
+      rstart[r] = s;
+      if (len > 0xFFFFFFF0ull) {
+        atomicMin(too_long, (ull)r);
+        rlen[r] = 0;
+        kc[r] = 0;
+      } else {
+        rlen[r] = (uint32_t)len;
+        kc[r] = len >= (uint64_t)K ? (uint32_t)(len - K + 1) : 0u;
+      }
+    }
+  }
+  if (!processed) {
+    rstart[r] = 0;
+    rlen[r] = 0;
+    kc[r] = 0;
+    atomicMin(first_unproc, (ull)r);
+  }
+  // assert(ptr[p] == '>') at the start of record r >= 1 when it is reached
+  if (r >= 1 && p0 < n) {
+    const uint64_t hp = j0 + 2 * r - 1;
+    if (hp < m) {
+      const uint64_t q = nl[hp] + 1;
+      if (q < n && fa[q] != '>') atomicMin(first_err, (ull)r);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-read rolling k-mer parse (one thread per read).
+
+__device__ __forceinline__ uint64_t load_word8(const uint8_t* fa, uint64_t n, uint64_t a) {
+  if (a + 8 <= n) return *reinterpret_cast<const uint64_t*>(fa + a);
+  uint64_t w = 0;
+  for (int i = 0; i < 8; i++)
+    if (a + i < n) w |= (uint64_t)fa[a + i] << (8 * i);
+  return w;
+}
+
+// Calls f(kmer) for every k-mer of bytes [s, s+len), in read order:
+// kmer = ((kmer << 2) | ((c >> 1) & 7)) & mask  (kmer_shift / char_to_el).
+template <class Key, class F>
+__device__ __forceinline__ void for_each_kmer(const uint8_t* fa, uint64_t n, uint64_t s,
+                                              uint32_t len, int K, Key mask, F&& f) {
+  if (len < (uint32_t)K) return;
+  const uint64_t e = s + len;
+  const bool aligned = (reinterpret_cast<uintptr_t>(fa) & 7) == 0;
+  Key km = 0;
+  uint64_t wa = s & ~7ull;
+  uint64_t w = aligned ? load_word8(fa, n, wa) : 0;
+  uint32_t i = 0;
+  for (uint64_t p = s; p < e; ++p, ++i) {
+    uint32_t c;
+    if (aligned) {
+      const uint64_t a = p & ~7ull;
+      if (a != wa) {
+        wa = a;
+        w = load_word8(fa, n, wa);
+      }
+      c = (uint32_t)(w >> ((p & 7) * 8)) & 0xFFu;
+    } else {
+      c = fa[p];
+    }
+    km = ((km << 2) | (Key)((c >> 1) & 7u)) & mask;
+    if (i + 1 >= (uint32_t)K) f(km);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Partition into buckets by the top b1 bits.
+
+constexpr int PART_NT = 512;
+constexpr int B1_MAX = 14;
+
+template <class Key>
+__global__ __launch_bounds__(PART_NT) void k_bucket_hist(const uint8_t* fa, uint64_t n,
+                                                         const ull* rstart, const uint32_t* rlen,
+                                                         uint64_t R, int K, Key mask, int b1,
+                                                         int reads_per_block, ull* gcnt) {
+  __shared__ uint32_t h[1 << B1_MAX];
+  const int nb = 1 << b1;
+  for (int i = threadIdx.x; i < nb; i += PART_NT) h[i] = 0;
+  __syncthreads();
+  const uint64_t r0 = (uint64_t)blockIdx.x * reads_per_block;
+  uint64_t r1 = r0 + reads_per_block;
+  if (r1 > R) r1 = R;
+  const int bpos = 2 * K - b1;
+  for (uint64_t r = r0 + threadIdx.x; r < r1; r += PART_NT) {
+    for_each_kmer<Key>(fa, n, rstart[r], rlen[r], K, mask,
+                       [&](Key km) { atomicAdd(&h[key_digit<Key>(km, bpos, b1)], 1u); });
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb; i += PART_NT)
+    if (h[i]) atomicAdd(&gcnt[i], (ull)h[i]);
+}
+
+template <class Key>
+__global__ __launch_bounds__(PART_NT) void k_bucket_scatter(const uint8_t* fa, uint64_t n,
+                                                            const ull* rstart, const uint32_t* rlen,
+                                                            uint64_t R, int K, Key mask, int b1,
+                                                            int reads_per_block, const ull* boff,
+                                                            ull* bcur, Key* S) {
+  __shared__ uint32_t h[1 << B1_MAX];
+  const int nb = 1 << b1;
+  for (int i = threadIdx.x; i < nb; i += PART_NT) h[i] = 0;
+  __syncthreads();
+  const uint64_t r0 = (uint64_t)blockIdx.x * reads_per_block;
+  uint64_t r1 = r0 + reads_per_block;
+  if (r1 > R) r1 = R;
+  const int bpos = 2 * K - b1;
+  for (uint64_t r = r0 + threadIdx.x; r < r1; r += PART_NT) {
+    for_each_kmer<Key>(fa, n, rstart[r], rlen[r], K, mask,
+                       [&](Key km) { atomicAdd(&h[key_digit<Key>(km, bpos, b1)], 1u); });
+  }
+  __syncthreads();
+  // reserve this block's range in every bucket (offset relative to the bucket)
+  for (int i = threadIdx.x; i < nb; i += PART_NT)
+    if (h[i]) h[i] = (uint32_t)atomicAdd(&bcur[i], (ull)h[i]);
+  __syncthreads();
+  for (uint64_t r = r0 + threadIdx.x; r < r1; r += PART_NT) {
+    for_each_kmer<Key>(fa, n, rstart[r], rlen[r], K, mask, [&](Key km) {
+      const uint32_t b = key_digit<Key>(km, bpos, b1);
+      const uint32_t rel = atomicAdd(&h[b], 1u);
+      S[boff[b] + rel] = km;
+    });
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-bucket count: registers -> LDS counting sort by a 12-bit sub-digit in
+// rounds, sub-bin sort, run-length encode.
+
+constexpr int BS_NT = 512;
+constexpr int SUB_MAX_BITS = 12;
+constexpr int SUB_BINS = 1 << SUB_MAX_BITS;
+constexpr int MAX_ROUNDS = 48;
+constexpr int SMALL_BIN = 512;
+
+template <class Key>
+struct BucketCfg;
+template <>
+struct BucketCfg<uint64_t> {
+  static constexpr int KPT = 40;    // keys per thread held in registers
+  static constexpr int RC = 8192;   // keys per LDS round (64 KB)
+};
+template <>
+struct BucketCfg<u128> {
+  static constexpr int KPT = 24;
+  static constexpr int RC = 4096;
+};
+
+template <class Key>
+__device__ void lds_bitonic(Key* L, int P) {
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < (P >> 1); i += BS_NT) {
+        const int idx = 2 * j * (i / j) + (i % j);
+        const int ptn = idx + j;
+        const bool up = (idx & k) == 0;
+        Key a = L[idx], b = L[ptn];
+        if ((a > b) == up) {
+          L[idx] = b;
+          L[ptn] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Tasks: [tstart[i], tstart[i] + tcount[i]) of S.  Writes the task's unique
+// (key, count) pairs at the front of its range (S in place, C alongside) and
+// tu[i] = number of pairs, or tu[i] = -1 when the task does not fit (then the
+// task index is appended to ovf_list).
+template <class Key>
+__global__ __launch_bounds__(BS_NT) void k_bucket_sort(Key* S, int32_t* C, const ull* tstart,
+                                                       const ull* tcount, uint64_t ntasks,
+                                                       int dpos, int dbits, long long* tu,
+                                                       ull* ovf_count, ull* ovf_list) {
+  constexpr int KPT = BucketCfg<Key>::KPT;
+  constexpr int RC = BucketCfg<Key>::RC;
+  __shared__ Key L[RC];
+  __shared__ uint32_t hs[SUB_BINS];
+  __shared__ uint32_t he[SUB_BINS];
+  __shared__ int r_lo[MAX_ROUNDS], r_hi[MAX_ROUNDS];
+  __shared__ int n_rounds, bad, use_bitonic;
+  __shared__ ull run_base;
+
+  const uint64_t task = blockIdx.x;
+  if (task >= ntasks) return;
+  const uint64_t start = tstart[task];
+  const uint64_t n = tcount[task];
+  if (n == 0) {
+    if (threadIdx.x == 0) tu[task] = 0;
+    return;
+  }
+  if (n > (uint64_t)KPT * BS_NT) {
+    if (threadIdx.x == 0) {
+      tu[task] = -1;
+      ovf_list[atomicAdd(ovf_count, 1ull)] = task;
+    }
+    return;
+  }
+  const int nbins = 1 << dbits;
+  // keys tid, tid + BS_NT, ... of the task; nv of them are valid
+  const int nv = (int)((n > threadIdx.x) ? (n - threadIdx.x - 1) / BS_NT + 1 : 0);
+  Key keys[KPT];
+  const Key* src = S + start + threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < KPT; i++) keys[i] = i < nv ? src[i * BS_NT] : Key(0);
+  for (int i = threadIdx.x; i < nbins; i += BS_NT) hs[i] = 0;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < KPT; i++)
+    if (i < nv) atomicAdd(&hs[key_digit<Key>(keys[i], dpos, dbits)], 1u);
+  __syncthreads();
+  // exclusive scan of hs over nbins (BPT bins per thread) -> hs = starts, he = ends
+  {
+    constexpr int BPT = SUB_BINS / BS_NT;
+    uint32_t v[BPT];
+    ull s = 0, mx = 0;
+#pragma unroll
+    for (int i = 0; i < BPT; i++) {
+      const int b = threadIdx.x * BPT + i;
+      v[i] = b < nbins ? hs[b] : 0u;
+      s += v[i];
+      mx = v[i] > mx ? v[i] : mx;
+    }
+    ull ex = block_excl_scan<BS_NT>(s);
+    mx = block_max<BS_NT>(mx);
+#pragma unroll
+    for (int i = 0; i < BPT; i++) {
+      const int b = threadIdx.x * BPT + i;
+      if (b < nbins) {
+        hs[b] = (uint32_t)ex;
+        ex += v[i];
+        he[b] = (uint32_t)ex;
+      }
+    }
+    if (threadIdx.x == 0) {
+      use_bitonic = mx > (ull)SMALL_BIN;
+      bad = mx > (ull)RC;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && !bad) {
+    // greedy rounds over bins: each round holds <= RC keys
+    int nr = 0, d = 0;
+    uint32_t s = 0;
+    while (d < nbins) {
+      if (nr >= MAX_ROUNDS) {
+        bad = 1;
+        break;
+      }
+      // largest e >= d with he[e] - s <= RC (he is non-decreasing)
+      int lo = d, hi = nbins - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (he[mid] - s <= (uint32_t)RC) lo = mid; else hi = mid - 1;
+      }
+      r_lo[nr] = d;
+      r_hi[nr] = lo + 1;
+      nr++;
+      s = he[lo];
+      d = lo + 1;
+    }
+    n_rounds = nr;
+    run_base = 0;
+  }
+  __syncthreads();
+  if (bad) {
+    if (threadIdx.x == 0) {
+      tu[task] = -1;
+      ovf_list[atomicAdd(ovf_count, 1ull)] = task;
+    }
+    return;
+  }
+  const int nr = n_rounds;
+  for (int rd = 0; rd < nr; rd++) {
+    const int blo = r_lo[rd], bhi = r_hi[rd];
+    const uint32_t pbase = blo ? he[blo - 1] : 0u;
+    const int m = (int)(he[bhi - 1] - pbase);
+    if (m == 0) continue;  // uniform: every thread reads the same LDS values
+#pragma unroll
+    for (int i = 0; i < KPT; i++) {
+      if (i < nv) {
+        const int d = (int)key_digit<Key>(keys[i], dpos, dbits);
+        if (d >= blo && d < bhi) {
+          const uint32_t pos = atomicAdd(&hs[d], 1u) - pbase;
+          L[pos] = keys[i];
+        }
+      }
+    }
+    __syncthreads();
+    if (use_bitonic) {
+      int P = 1;
+      while (P < m) P <<= 1;
+      for (int i = m + threadIdx.x; i < P; i += BS_NT) L[i] = ~Key(0);
+      __syncthreads();
+      lds_bitonic<Key>(L, P);
+    } else {
+      for (int d = blo + threadIdx.x; d < bhi; d += BS_NT) {
+        const int a = (int)((d ? he[d - 1] : 0u) - pbase);
+        const int e = (int)(he[d] - pbase);
+        for (int i = a + 1; i < e; i++) {
+          const Key v = L[i];
+          int j = i - 1;
+          while (j >= a && L[j] > v) {
+            L[j + 1] = L[j];
+            j--;
+          }
+          L[j + 1] = v;
+        }
+      }
+    }
+    __syncthreads();
+    // run-length encode L[0, m)
+    const int c = (m + BS_NT - 1) / BS_NT;
+    const int p0 = threadIdx.x * c;
+    int p1 = p0 + c;
+    if (p1 > m) p1 = m;
+    ull starts = 0;
+    for (int p = p0; p < p1; p++) starts += (p == 0 || L[p] != L[p - 1]);
+    ull ridx = block_excl_scan<BS_NT>(starts);
+    const ull rb = run_base;
+    for (int p = p0; p < p1; p++) {
+      if (p == 0 || L[p] != L[p - 1]) {
+        int q = p + 1;
+        while (q < m && L[q] == L[p]) q++;
+        const uint64_t o = start + rb + ridx;
+        S[o] = L[p];
+        C[o] = q - p;
+        ridx++;
+      }
+    }
+    const ull my_runs = starts;
+    const ull all = block_sum<BS_NT>(my_runs);
+    if (threadIdx.x == 0) run_base = rb + all;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tu[task] = (long long)run_base;
+}
+
+// All keys of the task are identical (no bits left to split on).
+template <class Key>
+__global__ void k_single_key(int32_t* C, const ull* tstart, const ull* tcount, uint64_t ntasks,
+                             long long* tu) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntasks) return;
+  if (tcount[i] == 0) {
+    tu[i] = 0;
+    return;
+  }
+  C[tstart[i]] = (int32_t)(uint32_t)tcount[i];
+  tu[i] = 1;
+}
+
+// Split oversize tasks by the pbits-wide digit at bit pos (host-driven).
+constexpr int PT_NT = 256;
+constexpr int PT_PER = 4096;
+
+template <class Key>
+__global__ __launch_bounds__(PT_NT) void k_part_hist(const Key* S, const ull* tstart,
+                                                     const ull* tcount, const uint32_t* blk_task,
+                                                     const ull* blk_off, int pos, int pbits,
+                                                     ull* th) {
+  __shared__ uint32_t h[256];
+  const int nb = 1 << pbits;
+  for (int i = threadIdx.x; i < nb; i += PT_NT) h[i] = 0;
+  __syncthreads();
+  const uint32_t t = blk_task[blockIdx.x];
+  const uint64_t s = tstart[t], n = tcount[t];
+  const uint64_t off = blk_off[blockIdx.x];
+  for (uint64_t i = off + threadIdx.x; i < n && i < off + PT_PER; i += PT_NT)
+    atomicAdd(&h[key_digit<Key>(S[s + i], pos, pbits)], 1u);
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb; i += PT_NT)
+    if (h[i]) atomicAdd(&th[(uint64_t)t * 256 + i], (ull)h[i]);
+}
+
+template <class Key>
+__global__ __launch_bounds__(PT_NT) void k_part_scatter(const Key* S, const ull* tstart,
+                                                        const ull* tcount, const uint32_t* blk_task,
+                                                        const ull* blk_off, int pos, int pbits,
+                                                        ull* tcur, Key* T) {
+  __shared__ uint32_t h[256];
+  __shared__ ull base[256];
+  const int nb = 1 << pbits;
+  for (int i = threadIdx.x; i < nb; i += PT_NT) h[i] = 0;
+  __syncthreads();
+  const uint32_t t = blk_task[blockIdx.x];
+  const uint64_t s = tstart[t], n = tcount[t];
+  const uint64_t off = blk_off[blockIdx.x];
+  for (uint64_t i = off + threadIdx.x; i < n && i < off + PT_PER; i += PT_NT)
+    atomicAdd(&h[key_digit<Key>(S[s + i], pos, pbits)], 1u);
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb; i += PT_NT)
+    base[i] = h[i] ? atomicAdd(&tcur[(uint64_t)t * 256 + i], (ull)h[i]) : 0ull;
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb; i += PT_NT) h[i] = 0;
+  __syncthreads();
+  for (uint64_t i = off + threadIdx.x; i < n && i < off + PT_PER; i += PT_NT) {
+    const Key k = S[s + i];
+    const uint32_t d = key_digit<Key>(k, pos, pbits);
+    const uint32_t r = atomicAdd(&h[d], 1u);
+    T[s + base[d] + r] = k;
+  }
+}
+
+template <class Key>
+__global__ void k_copy_tasks(Key* S, const Key* T, const ull* tstart, const ull* tcount,
+                             const uint32_t* blk_task, const ull* blk_off) {
+  const uint32_t t = blk_task[blockIdx.x];
+  const uint64_t s = tstart[t], n = tcount[t];
+  const uint64_t off = blk_off[blockIdx.x];
+  for (uint64_t i = off + threadIdx.x; i < n && i < off + PT_PER; i += blockDim.x) S[s + i] = T[s + i];
+}
+
+// Dense table from per-leaf results (leaves in key order).
+template <class Key>
+__global__ void k_compact(const Key* S, const int32_t* C, const ull* lstart, const long long* lu,
+                          const ull* loff, uint64_t nleaves, Key* TK, int32_t* TC) {
+  const uint64_t b = blockIdx.x;
+  if (b >= nleaves) return;
+  const long long u = lu[b];
+  const uint64_t s = lstart[b], o = loff[b];
+  for (long long i = threadIdx.x; i < u; i += blockDim.x) {
+    TK[o + i] = S[s + i];
+    TC[o + i] = C[s + i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cutoff (kmer_histogram.cpp:84-140).
+
+constexpr int CUT_NT = 1024;
+constexpr int T_MAX = 4096;
+
+__global__ __launch_bounds__(CUT_NT) void k_count_hist(const int32_t* TC, uint64_t U, int t, ull* hist) {
+  __shared__ uint32_t h[T_MAX];
+  for (int i = threadIdx.x; i < t; i += CUT_NT) h[i] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER;
+  for (uint64_t i = base + threadIdx.x; i < U && i < base + SCAN_PER; i += CUT_NT) {
+    const int v = TC[i];
+    if (v >= 0 && v < t) atomicAdd(&h[v], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < t; i += CUT_NT)
+    if (h[i]) atomicAdd(&hist[i], (ull)h[i]);
+}
+
+// min_bucket = first argmin over hist[1..t-1]; an empty range gives t.
+__global__ void k_min_bucket(const ull* hist, int t, int* mb) {
+  if (threadIdx.x || blockIdx.x) return;
+  int best = t;
+  if (t > 1) {
+    best = 1;
+    for (int i = 2; i < t; i++)
+      if (hist[i] < hist[best]) best = i;
+  }
+  *mb = best;
+}
+
+__global__ __launch_bounds__(CUT_NT) void k_kept_count(const int32_t* TC, uint64_t U, const int* mb,
+                                                      ull* bsum) {
+  const int thr = *mb;
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER + (uint64_t)threadIdx.x * 4;
+  ull c = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    if (base + i < U) c += TC[base + i] >= thr;
+  c = block_sum<CUT_NT>(c);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = c;
+}
+
+template <class Key>
+__global__ __launch_bounds__(CUT_NT) void k_kept_write(const Key* TK, const int32_t* TC, uint64_t U,
+                                                      const int* mb, const ull* boffs, Key* KK,
+                                                      int32_t* KC) {
+  const int thr = *mb;
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER + (uint64_t)threadIdx.x * 4;
+  int32_t cv[4];
+  ull c = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    cv[i] = base + i < U ? TC[base + i] : 0;
+    c += (base + i < U) && cv[i] >= thr;
+  }
+  ull o = block_excl_scan<CUT_NT>(c) + boffs[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    if (base + i < U && cv[i] >= thr) {
+      KK[o] = TK[base + i];
+      KC[o] = cv[i];
+      o++;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Macro nodes (mn_construct.cpp:212-477).
+//
+// The kept table KK[0..V) is ascending.  pred(x) = x >> 2 is non-decreasing
+// over it; within quarter a (first base a, indices [q[a], q[a+1])) succ(x) =
+// x & SUCC_MASK is ascending.  Node keys are the union of these five sorted
+// lists.  Range g covers node keys [s_g, s_{g+1}) with s_g = pred(KK[g*PC]).
+
+constexpr int MN_NT = 512;
+constexpr int MN_PC = 1024;    // pred entries per range
+constexpr int MN_ECAP = 4096;  // LDS capacity (entries of all five lists)
+
+struct MnPlan {
+  ull lo[5];
+  ull hi[5];
+  ull base;
+  ull e;
+};
+
+template <class Key>
+__device__ __forceinline__ Key mn_pred(Key x) { return x >> 2; }
+
+template <class Key>
+__global__ void k_mn_quarters(const Key* KK, uint64_t V, int K, ull* q) {
+  if (threadIdx.x >= 5 || blockIdx.x) return;
+  const int a = threadIdx.x;
+  if (a == 0) { q[0] = 0; return; }
+  if (a == 4) { q[4] = V; return; }
+  const Key lim = Key(a) << (2 * K - 2);
+  uint64_t lo = 0, hi = V;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (KK[mid] < lim) lo = mid + 1; else hi = mid;
+  }
+  q[a] = lo;
+}
+
+template <class Key>
+__device__ __forceinline__ uint64_t lb_pred(const Key* KK, uint64_t lo, uint64_t hi, Key x) {
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (mn_pred(KK[mid]) < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+template <class Key>
+__device__ __forceinline__ uint64_t lb_succ(const Key* KK, uint64_t lo, uint64_t hi, Key x, Key m) {
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if ((KK[mid] & m) < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+template <class Key>
+__global__ void k_mn_plan(const Key* KK, uint64_t V, int K, const ull* q, uint64_t G, MnPlan* plan,
+                          ull* maxe) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  const Key msk = key_mask<Key>(2 * (K - 1));
+  const bool first = g == 0, last = g + 1 == G;
+  const Key s_lo = first ? Key(0) : mn_pred(KK[g * MN_PC]);
+  const Key s_hi = last ? Key(0) : mn_pred(KK[(g + 1) * MN_PC]);
+  MnPlan p;
+  p.lo[0] = first ? 0 : lb_pred(KK, 0, V, s_lo);
+  p.hi[0] = last ? V : lb_pred(KK, 0, V, s_hi);
+  ull base = p.lo[0], e = p.hi[0] - p.lo[0];
+  for (int a = 0; a < 4; a++) {
+    p.lo[a + 1] = first ? q[a] : lb_succ(KK, q[a], q[a + 1], s_lo, msk);
+    p.hi[a + 1] = last ? q[a + 1] : lb_succ(KK, q[a], q[a + 1], s_hi, msk);
+    base += p.lo[a + 1] - q[a];
+    e += p.hi[a + 1] - p.lo[a + 1];
+  }
+  p.base = base;
+  p.e = e;
+  plan[g] = p;
+  atomicMax(maxe, e);
+}
+
+// Node record as written to memory (matches pk_mnode).
+struct MnRec {
+  ull key_lo, key_hi;
+  int32_t pre[4];
+  int32_t suf[4];
+};
+
+template <class Key>
+__device__ __forceinline__ void set_key(MnRec& r, Key k);
+template <>
+__device__ __forceinline__ void set_key<uint64_t>(MnRec& r, uint64_t k) {
+  r.key_lo = k;
+  r.key_hi = 0;
+}
+template <>
+__device__ __forceinline__ void set_key<u128>(MnRec& r, u128 k) {
+  r.key_lo = (ull)k;
+  r.key_hi = (ull)(k >> 64);
+}
+
+// List view: 5 sorted lists of node keys, either staged in LDS (lk) or
+// read from KK through pred/succ.
+template <class Key, bool GLOBAL>
+struct MnLists {
+  const Key* KK;
+  Key msk;
+  const Key* lk;       // LDS copy (GLOBAL == false)
+  ull off[5];          // list start (index into lk, or into KK)
+  ull len[5];
+  __device__ __forceinline__ Key at(int l, ull i) const {
+    if (GLOBAL) {
+      const Key x = KK[off[l] + i];
+      return l == 0 ? mn_pred(x) : (x & msk);
+    }
+    return lk[off[l] + i];
+  }
+  __device__ __forceinline__ ull lb(int l, Key x) const {
+    ull lo = 0, hi = len[l];
+    while (lo < hi) {
+      ull mid = (lo + hi) >> 1;
+      if (at(l, mid) < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+  }
+  __device__ __forceinline__ ull ub(int l, Key x) const {
+    ull lo = 0, hi = len[l];
+    while (lo < hi) {
+      ull mid = (lo + hi) >> 1;
+      if (at(l, mid) <= x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+  }
+};
+
+// One block per range.  GLOBAL == false: ranges with e <= MN_ECAP, lists in
+// LDS.  GLOBAL == true: the remaining ranges, lists and rank flags in global
+// memory (gflags, indexed from plan.base).
+template <class Key, bool GLOBAL>
+__global__ __launch_bounds__(MN_NT) void k_mn_build(const Key* KK, const int32_t* KC, int K,
+                                                    const MnPlan* plan, uint64_t G, MnRec* NT,
+                                                    ull* ng, uint32_t* gflags) {
+  __shared__ Key lk[GLOBAL ? 1 : MN_ECAP];
+  __shared__ uint32_t lflag[GLOBAL ? 1 : MN_ECAP];
+  __shared__ ull chunk_base;
+  const uint64_t g = blockIdx.x;
+  if (g >= G) return;
+  const MnPlan p = plan[g];
+  const bool big = p.e > (ull)MN_ECAP;
+  if (big != GLOBAL) return;
+  const ull E = p.e;
+  MnLists<Key, GLOBAL> L;
+  L.KK = KK;
+  L.msk = key_mask<Key>(2 * (K - 1));
+  L.lk = lk;
+  {
+    ull o = 0;
+    for (int l = 0; l < 5; l++) {
+      L.len[l] = p.hi[l] - p.lo[l];
+      L.off[l] = GLOBAL ? p.lo[l] : o;
+      o += L.len[l];
+    }
+  }
+  if (!GLOBAL) {
+    for (int l = 0; l < 5; l++) {
+      for (ull i = threadIdx.x; i < L.len[l]; i += MN_NT) {
+        const Key x = KK[p.lo[l] + i];
+        lk[L.off[l] + i] = l == 0 ? mn_pred(x) : (x & L.msk);
+      }
+    }
+    __syncthreads();
+  }
+  uint32_t* flags = GLOBAL ? gflags + p.base : lflag;
+  // pass 1: merged rank and node-start flag of every element
+  for (ull e = threadIdx.x; e < E; e += MN_NT) {
+    int l = 0;
+    ull j = e;
+    while (j >= L.len[l]) { j -= L.len[l]; l++; }
+    const Key x = L.at(l, j);
+    ull rank = j;
+    bool start = L.lb(l, x) == j;
+    for (int o = 0; o < 5; o++) {
+      if (o == l) continue;
+      const ull lbo = L.lb(o, x);
+      if (o < l) {
+        const ull ubo = L.ub(o, x);
+        rank += ubo;
+        if (ubo != lbo) start = false;
+      } else {
+        rank += lbo;
+      }
+    }
+    flags[rank] = start ? 1u : 0u;
+  }
+  __syncthreads();
+  // pass 2: exclusive scan of flags in rank order (chunks of MN_NT*4)
+  if (threadIdx.x == 0) chunk_base = 0;
+  __syncthreads();
+  for (ull c0 = 0; c0 < E; c0 += (ull)MN_NT * 4) {
+    const ull i0 = c0 + (ull)threadIdx.x * 4;
+    uint32_t v[4];
+    ull s = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      v[i] = i0 + i < E ? flags[i0 + i] : 0u;
+      s += v[i];
+    }
+    ull ex = block_excl_scan<MN_NT>(s) + chunk_base;
+    const ull tot = block_sum<MN_NT>(s);
+    // store "node index + 1" for starts, 0 otherwise
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      if (i0 + i < E) flags[i0 + i] = v[i] ? (uint32_t)(ex + 1) : 0u;
+      ex += v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) chunk_base += tot;
+    __syncthreads();
+  }
+  const ull nodes = chunk_base;
+  // pass 3: every node start assembles its record
+  for (ull e = threadIdx.x; e < E; e += MN_NT) {
+    int l = 0;
+    ull j = e;
+    while (j >= L.len[l]) { j -= L.len[l]; l++; }
+    const Key x = L.at(l, j);
+    if (L.lb(l, x) != j) continue;
+    ull rank = j;
+    ull lbs[5], ubs[5];
+    bool start = true;
+    for (int o = 0; o < 5; o++) {
+      lbs[o] = L.lb(o, x);
+      ubs[o] = L.ub(o, x);
+      if (o < l) {
+        rank += ubs[o];
+        if (ubs[o] != lbs[o]) start = false;
+      } else if (o > l) {
+        rank += lbs[o];
+      }
+    }
+    if (!start) continue;
+    const uint32_t idx1 = flags[rank];
+    MnRec rec;
+    set_key<Key>(rec, x);
+    for (int b = 0; b < 4; b++) rec.pre[b] = rec.suf[b] = 0;
+    for (ull i = lbs[0]; i < ubs[0]; i++) {
+      const ull gi = p.lo[0] + i;
+      rec.suf[(int)(KK[gi] & Key(3))] = KC[gi];
+    }
+    for (int a = 0; a < 4; a++)
+      if (ubs[a + 1] > lbs[a + 1]) rec.pre[a] = KC[p.lo[a + 1] + lbs[a + 1]];
+    NT[p.base + idx1 - 1] = rec;
+  }
+  if (threadIdx.x == 0) ng[g] = nodes;
+}
+
+__global__ void k_mn_compact(const MnRec* NT, const MnPlan* plan, const ull* ng, const ull* noff,
+                             uint64_t G, MnRec* out) {
+  const uint64_t g = blockIdx.x;
+  if (g >= G) return;
+  const ull b = plan[g].base, o = noff[g], c = ng[g];
+  for (ull i = threadIdx.x; i < c; i += blockDim.x) out[o + i] = NT[b + i];
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+enum BufId {
+  B_FASTA, B_BLK, B_BLKOFF, B_NL, B_RSTART, B_RLEN, B_KC, B_KOFS, B_SMALL, B_BCNT, B_BOFF, B_BCUR,
+  B_S, B_C, B_TU, B_UOFF, B_TK, B_TC, B_HIST, B_KBS, B_KBOFF, B_KK, B_KC2, B_MNQ, B_MNPLAN,
+  B_MNNT, B_MNNG, B_MNNOFF, B_MNFLAG, B_NODES, B_TMP, B_OVF, B_PT0, B_PT1, B_PT2, B_PT3, B_PT4,
+  B_PT5, B_PT6, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
+};
+
+}  // namespace
+
+struct pk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t budget = 0;
+  uint64_t held = 0;
+  std::string err;
+  DevBuf buf[B_COUNT];
+  hipEvent_t ev[16] = {};
+  pk_timing timing{};
+  // state of the current table
+  int k = 0;
+  int words = 1;
+  uint64_t U = 0, V = 0, M = 0, N = 0, R = 0;
+  bool have_table = false, have_kept = false, have_nodes = false;
+  int t = 0, mb = 0;
+
+  int fail(int code, const std::string& what) {
+    err = what;
+    return code;
+  }
+  int hip_fail(hipError_t e, const char* what) {
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return PK_EHIP;
+  }
+};
+
+#define PK_HIP(ctx, x)                                   \
+  do {                                                   \
+    hipError_t e_ = (x);                                 \
+    if (e_ != hipSuccess) return (ctx)->hip_fail(e_, #x); \
+  } while (0)
+
+#define PK_TRY(x)          \
+  do {                     \
+    int rc_ = (x);         \
+    if (rc_) return rc_;   \
+  } while (0)
+
+namespace {
+
+template <class T>
+int need(pk_ctx* c, int id, uint64_t count, T** out) {
+  DevBuf& b = c->buf[id];
+  const uint64_t bytes = count * sizeof(T) + 256;
+  if (b.cap < bytes) {
+    if (b.p) {
+      hipError_t e = hipFree(b.p);
+      (void)e;
+      c->held -= b.cap;
+      b.p = nullptr;
+      b.cap = 0;
+    }
+    if (c->budget && c->held + bytes > c->budget)
+      return c->fail(PK_ENOMEM, "device budget exceeded");
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return c->fail(PK_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    b.p = p;
+    b.cap = bytes;
+    c->held += bytes;
+  }
+  *out = reinterpret_cast<T*>(b.p);
+  return PK_OK;
+}
+
+inline unsigned grid_for(uint64_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
+
+int fill_u64(pk_ctx* c, ull* p, ull v, uint64_t n) {
+  if (!n) return PK_OK;
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, p, v, n);
+  PK_HIP(c, hipGetLastError());
+  return PK_OK;
+}
+
+// out[0..n) = exclusive scan of in, out[n] = total.  level selects scratch.
+template <class T>
+int scan_excl(pk_ctx* c, const T* in, ull* out, uint64_t n, int level = 0) {
+  if (n == 0) {
+    return fill_u64(c, out, 0, 1);
+  }
+  if (level >= 8) return c->fail(PK_EINVAL, "scan too deep");
+  const uint64_t nb = (n + SCAN_PER - 1) / SCAN_PER;
+  if (nb == 1) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_scan_apply<T>), dim3(1), dim3(SCAN_NT), 0, c->stream, in,
+                       out, (const ull*)nullptr, n);
+    PK_HIP(c, hipGetLastError());
+    return PK_OK;
+  }
+  ull* sums;
+  PK_TRY(need(c, B_SCAN0 + level, nb + 1, &sums));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_scan_reduce<T>), dim3((unsigned)nb), dim3(SCAN_NT), 0,
+                     c->stream, in, sums, n);
+  PK_HIP(c, hipGetLastError());
+  PK_TRY(scan_excl<ull>(c, sums, sums, nb, level + 1));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_scan_apply<T>), dim3((unsigned)nb), dim3(SCAN_NT), 0,
+                     c->stream, in, out, (const ull*)sums, n);
+  PK_HIP(c, hipGetLastError());
+  // total = sums[nb]
+  hipLaunchKernelGGL(k_copy_u64, dim3(1), dim3(1), 0, c->stream, out + n, sums + nb);
+  PK_HIP(c, hipGetLastError());
+  return PK_OK;
+}
+
+int d2h(pk_ctx* c, void* dst, const void* src, size_t bytes) {
+  PK_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  PK_HIP(c, hipStreamSynchronize(c->stream));
+  return PK_OK;
+}
+
+int choose_b1(uint64_t N, int K, int cap_keys) {
+  int b1 = 0;
+  while (b1 < B1_MAX && b1 < 2 * K && (N >> b1) > (uint64_t)(cap_keys / 2)) b1++;
+  return b1;
+}
+
+// Sub-digit window of a task whose keys share their top `fixed` bits.
+void sub_digit(int K, int fixed, int* dpos, int* dbits) {
+  const int rem = 2 * K - fixed;
+  if (rem <= 0) {
+    *dpos = 0;
+    *dbits = 0;
+    return;
+  }
+  const int nb = rem < SUB_MAX_BITS ? rem : SUB_MAX_BITS;
+  *dpos = rem - nb;
+  *dbits = nb;
+}
+
+struct Leaf {
+  ull start;
+  long long u;
+};
+
+// Recursively split oversize tasks; returns the leaves (in key order) for
+// each input task.
+template <class Key>
+int resolve_overflow(pk_ctx* c, Key* S, int32_t* C, int K, int fixed_bits,
+                     const std::vector<ull>& starts, const std::vector<ull>& counts,
+                     std::vector<std::vector<Leaf>>& leaves) {
+  constexpr int CAP = BucketCfg<Key>::KPT * BS_NT;
+  leaves.assign(starts.size(), {});
+  const int rem = 2 * K - fixed_bits;
+  if (rem <= 0) {
+    // all keys of every task are identical
+    const uint64_t nt = starts.size();
+    ull *ts, *tc;
+    long long* tu;
+    PK_TRY(need(c, B_PT0, nt, &ts));
+    PK_TRY(need(c, B_PT1, nt, &tc));
+    PK_TRY(need(c, B_PT2, nt, &tu));
+    PK_HIP(c, hipMemcpyAsync(ts, starts.data(), nt * 8, hipMemcpyHostToDevice, c->stream));
+    PK_HIP(c, hipMemcpyAsync(tc, counts.data(), nt * 8, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_single_key<Key>), dim3(grid_for(nt, 256)), dim3(256), 0,
+                       c->stream, C, ts, tc, nt, tu);
+    PK_HIP(c, hipGetLastError());
+    std::vector<long long> hu(nt);
+    PK_TRY(d2h(c, hu.data(), tu, nt * 8));
+    for (size_t i = 0; i < nt; i++) leaves[i].push_back(Leaf{starts[i], hu[i]});
+    return PK_OK;
+  }
+  // digit width for this level
+  ull maxn = 0;
+  for (auto n : counts) maxn = std::max<ull>(maxn, n);
+  int pbits = 1;
+  while (pbits < 8 && (maxn >> pbits) > (ull)(CAP / 2)) pbits++;
+  if (pbits > rem) pbits = rem;
+  const int pos = rem - pbits;
+  // block map
+  std::vector<uint32_t> bt;
+  std::vector<ull> bo;
+  for (size_t t = 0; t < starts.size(); t++)
+    for (ull o = 0; o < counts[t]; o += PT_PER) {
+      bt.push_back((uint32_t)t);
+      bo.push_back(o);
+    }
+  const uint64_t nt = starts.size(), nblk = bt.size();
+  ull *ts, *tc, *bo_d, *th;
+  uint32_t* bt_d;
+  Key* T;
+  PK_TRY(need(c, B_PT0, nt, &ts));
+  PK_TRY(need(c, B_PT1, nt, &tc));
+  PK_TRY(need(c, B_PT3, nblk, &bt_d));
+  PK_TRY(need(c, B_PT4, nblk, &bo_d));
+  PK_TRY(need(c, B_PT5, nt * 256, &th));
+  PK_TRY(need(c, B_TMP, c->N, &T));
+  PK_HIP(c, hipMemcpyAsync(ts, starts.data(), nt * 8, hipMemcpyHostToDevice, c->stream));
+  PK_HIP(c, hipMemcpyAsync(tc, counts.data(), nt * 8, hipMemcpyHostToDevice, c->stream));
+  PK_HIP(c, hipMemcpyAsync(bt_d, bt.data(), nblk * 4, hipMemcpyHostToDevice, c->stream));
+  PK_HIP(c, hipMemcpyAsync(bo_d, bo.data(), nblk * 8, hipMemcpyHostToDevice, c->stream));
+  PK_TRY(fill_u64(c, th, 0, nt * 256));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_part_hist<Key>), dim3((unsigned)nblk), dim3(PT_NT), 0,
+                     c->stream, S, ts, tc, bt_d, bo_d, pos, pbits, th);
+  PK_HIP(c, hipGetLastError());
+  std::vector<ull> hh(nt * 256);
+  PK_TRY(d2h(c, hh.data(), th, nt * 256 * 8));
+  std::vector<ull> cur(nt * 256, 0);
+  std::vector<ull> sub_start, sub_count;
+  std::vector<std::pair<size_t, int>> owner;  // (task, digit)
+  for (size_t t = 0; t < nt; t++) {
+    ull o = 0;
+    for (int d = 0; d < (1 << pbits); d++) {
+      cur[t * 256 + d] = o;
+      if (hh[t * 256 + d]) {
+        sub_start.push_back(starts[t] + o);
+        sub_count.push_back(hh[t * 256 + d]);
+        owner.push_back({t, d});
+      }
+      o += hh[t * 256 + d];
+    }
+  }
+  PK_HIP(c, hipMemcpyAsync(th, cur.data(), nt * 256 * 8, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_part_scatter<Key>), dim3((unsigned)nblk), dim3(PT_NT), 0,
+                     c->stream, S, ts, tc, bt_d, bo_d, pos, pbits, th, T);
+  PK_HIP(c, hipGetLastError());
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_copy_tasks<Key>), dim3((unsigned)nblk), dim3(PT_NT), 0,
+                     c->stream, S, T, ts, tc, bt_d, bo_d);
+  PK_HIP(c, hipGetLastError());
+  // sort the sub-tasks that fit
+  const int fixed2 = fixed_bits + pbits;
+  int dpos, dbits;
+  sub_digit(K, fixed2, &dpos, &dbits);
+  const uint64_t ns = sub_start.size();
+  ull *ss, *sc, *ovc, *ovl;
+  long long* su;
+  PK_TRY(need(c, B_PT6, ns * 3 + 2, &ss));
+  sc = ss + ns;
+  su = reinterpret_cast<long long*>(ss + 2 * ns);
+  PK_TRY(need(c, B_OVF, ns + 1, &ovl));
+  PK_TRY(need(c, B_PT2, 1, &ovc));
+  PK_HIP(c, hipMemcpyAsync(ss, sub_start.data(), ns * 8, hipMemcpyHostToDevice, c->stream));
+  PK_HIP(c, hipMemcpyAsync(sc, sub_count.data(), ns * 8, hipMemcpyHostToDevice, c->stream));
+  PK_TRY(fill_u64(c, ovc, 0, 1));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bucket_sort<Key>), dim3((unsigned)ns), dim3(BS_NT), 0,
+                     c->stream, S, C, ss, sc, ns, dpos, dbits, su, ovc, ovl);
+  PK_HIP(c, hipGetLastError());
+  std::vector<long long> hu(ns);
+  PK_TRY(d2h(c, hu.data(), su, ns * 8));
+  std::vector<ull> o_start, o_count;
+  std::vector<size_t> o_idx;
+  for (size_t i = 0; i < ns; i++)
+    if (hu[i] < 0) {
+      o_start.push_back(sub_start[i]);
+      o_count.push_back(sub_count[i]);
+      o_idx.push_back(i);
+    }
+  std::vector<std::vector<Leaf>> sub_leaves(ns);
+  if (!o_start.empty()) {
+    std::vector<std::vector<Leaf>> rec;
+    PK_TRY(resolve_overflow<Key>(c, S, C, K, fixed2, o_start, o_count, rec));
+    for (size_t j = 0; j < o_idx.size(); j++) sub_leaves[o_idx[j]] = rec[j];
+  }
+  for (size_t i = 0; i < ns; i++) {
+    auto& dst = leaves[owner[i].first];
+    if (hu[i] >= 0) dst.push_back(Leaf{sub_start[i], hu[i]});
+    else dst.insert(dst.end(), sub_leaves[i].begin(), sub_leaves[i].end());
+  }
+  return PK_OK;
+}
+
+template <class Key>
+int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
+  constexpr int CAP = BucketCfg<Key>::KPT * BS_NT;
+  hipStream_t st = c->stream;
+  c->have_table = c->have_kept = c->have_nodes = false;
+  c->timing = pk_timing{};
+  PK_HIP(c, hipEventRecord(c->ev[0], st));
+  // 1. line index
+  const uint64_t nb = (n + NL_PER_BLOCK - 1) / NL_PER_BLOCK;
+  ull *blk, *blkoff, *small, *nl;
+  PK_TRY(need(c, B_BLK, nb, &blk));
+  PK_TRY(need(c, B_BLKOFF, nb + 1, &blkoff));
+  PK_TRY(need(c, B_SMALL, 16, &small));
+  // small[0] = first '>', [1] = first unprocessed, [2] = first format error,
+  // [3] = first over-long read, [4] = overflow count
+  PK_TRY(fill_u64(c, small, ~0ull, 4));
+  PK_TRY(fill_u64(c, small + 4, 0, 1));
+  if (nb) {
+    hipLaunchKernelGGL(k_nl_count, dim3((unsigned)nb), dim3(NL_NT), 0, st, fa, n, blk, small);
+    PK_HIP(c, hipGetLastError());
+  }
+  PK_TRY(scan_excl<ull>(c, blk, blkoff, nb));
+  ull hm[2];
+  PK_HIP(c, hipMemcpyAsync(&hm[0], blkoff + nb, 8, hipMemcpyDeviceToHost, st));
+  PK_HIP(c, hipMemcpyAsync(&hm[1], small, 8, hipMemcpyDeviceToHost, st));
+  PK_HIP(c, hipStreamSynchronize(st));
+  const uint64_t m = hm[0];
+  const uint64_t p0 = hm[1] == ~0ull ? n : hm[1];
+  PK_TRY(need(c, B_NL, m + 1, &nl));
+  if (nb) {
+    hipLaunchKernelGGL(k_nl_write, dim3((unsigned)nb), dim3(NL_NT), 0, st, fa, n, blkoff, nl);
+    PK_HIP(c, hipGetLastError());
+  }
+  // 2. read table
+  const uint64_t ncand = m / 2 + 2;
+  ull *rstart, *kofs;
+  uint32_t *rlen, *kc;
+  PK_TRY(need(c, B_RSTART, ncand, &rstart));
+  PK_TRY(need(c, B_RLEN, ncand, &rlen));
+  PK_TRY(need(c, B_KC, ncand, &kc));
+  PK_TRY(need(c, B_KOFS, ncand + 1, &kofs));
+  hipLaunchKernelGGL(k_reads, dim3(grid_for(ncand, 256)), dim3(256), 0, st, fa, n, nl, m, p0, K,
+                     ncand, rstart, rlen, kc, small + 1, small + 2, small + 3);
+  PK_HIP(c, hipGetLastError());
+  PK_TRY(scan_excl<uint32_t>(c, kc, kofs, ncand));
+  ull hs[4];
+  PK_HIP(c, hipMemcpyAsync(hs, small + 1, 24, hipMemcpyDeviceToHost, st));
+  PK_HIP(c, hipMemcpyAsync(&hs[3], kofs + ncand, 8, hipMemcpyDeviceToHost, st));
+  PK_HIP(c, hipStreamSynchronize(st));
+  const uint64_t R = std::min<uint64_t>(hs[0], ncand);
+  if (hs[1] != ~0ull && hs[1] <= R)
+    return c->fail(PK_EFORMAT, "input is not strict single-line FASTA (record " +
+                                   std::to_string(hs[1]) + " does not start with '>')");
+  if (hs[2] != ~0ull && hs[2] < R) return c->fail(PK_EINVAL, "read longer than 4 GiB");
+  const uint64_t N = hs[3];
+  c->R = R;
+  c->N = N;
+  c->k = K;
+  c->words = sizeof(Key) / 8;
+  c->timing.n_instances = N;
+  c->timing.n_reads = R;
+  PK_HIP(c, hipEventRecord(c->ev[1], st));
+  Key *S, *TK;
+  int32_t *C, *TC;
+  if (N == 0) {
+    c->U = 0;
+    PK_TRY(need(c, B_TK, 1, &TK));
+    PK_TRY(need(c, B_TC, 1, &TC));
+    c->have_table = true;
+    return PK_OK;
+  }
+  // 3. partition
+  const Key mask = key_mask<Key>(2 * K);
+  const int b1 = choose_b1(N, K, CAP);
+  const uint64_t B = 1ull << b1;
+  ull *bcnt, *boff, *bcur;
+  PK_TRY(need(c, B_BCNT, B, &bcnt));
+  PK_TRY(need(c, B_BOFF, B + 1, &boff));
+  PK_TRY(need(c, B_BCUR, B, &bcur));
+  PK_TRY(need(c, B_S, N, &S));
+  PK_TRY(need(c, B_C, N, &C));
+  PK_TRY(fill_u64(c, bcnt, 0, B));
+  PK_TRY(fill_u64(c, bcur, 0, B));
+  const int rpb = 2048;  // reads per block
+  const unsigned pgrid = grid_for(R, rpb);
+  c->timing.bucket_bits = b1;
+  PK_HIP(c, hipEventRecord(c->ev[2], st));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bucket_hist<Key>), dim3(pgrid), dim3(PART_NT), 0, st, fa, n,
+                     rstart, rlen, R, K, mask, b1, rpb, bcnt);
+  PK_HIP(c, hipGetLastError());
+  PK_HIP(c, hipEventRecord(c->ev[3], st));
+  PK_TRY(scan_excl<ull>(c, bcnt, boff, B));
+  PK_HIP(c, hipEventRecord(c->ev[4], st));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bucket_scatter<Key>), dim3(pgrid), dim3(PART_NT), 0, st,
+                     fa, n, rstart, rlen, R, K, mask, b1, rpb, boff, bcur, S);
+  PK_HIP(c, hipGetLastError());
+  PK_HIP(c, hipEventRecord(c->ev[5], st));
+  // 4. per-bucket count
+  long long* tu;
+  ull *ovl, *uoff;
+  PK_TRY(need(c, B_TU, B, &tu));
+  PK_TRY(need(c, B_OVF, B + 1, &ovl));
+  PK_TRY(need(c, B_UOFF, B + 1, &uoff));
+  int dpos, dbits;
+  sub_digit(K, b1, &dpos, &dbits);
+  PK_HIP(c, hipEventRecord(c->ev[6], st));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bucket_sort<Key>), dim3((unsigned)B), dim3(BS_NT), 0, st, S,
+                     C, boff, bcnt, B, dpos, dbits, tu, small + 4, ovl);
+  PK_HIP(c, hipGetLastError());
+  PK_HIP(c, hipEventRecord(c->ev[7], st));
+  ull novf = 0;
+  PK_TRY(d2h(c, &novf, small + 4, 8));
+  c->timing.n_split = (int32_t)novf;
+  PK_HIP(c, hipEventRecord(c->ev[8], st));
+  const ull* lstart = boff;
+  const long long* lu = tu;
+  uint64_t nleaves = B;
+  if (novf) {
+    // leaves = buckets, with each oversize bucket replaced by its split
+    std::vector<ull> hbo(B + 1), hbc(B);
+    std::vector<long long> htu(B);
+    PK_TRY(d2h(c, hbo.data(), boff, (B + 1) * 8));
+    PK_TRY(d2h(c, hbc.data(), bcnt, B * 8));
+    PK_TRY(d2h(c, htu.data(), tu, B * 8));
+    std::vector<ull> os, oc;
+    std::vector<size_t> ob;
+    for (uint64_t b = 0; b < B; b++)
+      if (htu[b] < 0) {
+        os.push_back(hbo[b]);
+        oc.push_back(hbc[b]);
+        ob.push_back(b);
+      }
+    std::vector<std::vector<Leaf>> rec;
+    PK_TRY(resolve_overflow<Key>(c, S, C, K, b1, os, oc, rec));
+    std::vector<ull> ls;
+    std::vector<long long> lus;
+    size_t j = 0;
+    for (uint64_t b = 0; b < B; b++) {
+      if (j < ob.size() && ob[j] == b) {
+        for (auto& lf : rec[j]) {
+          ls.push_back(lf.start);
+          lus.push_back(lf.u);
+        }
+        j++;
+      } else {
+        ls.push_back(hbo[b]);
+        lus.push_back(htu[b]);
+      }
+    }
+    nleaves = ls.size();
+    ull* lsd;
+    long long* lud;
+    PK_TRY(need(c, B_PT0, nleaves, &lsd));
+    PK_TRY(need(c, B_PT1, nleaves, &lud));
+    PK_TRY(need(c, B_UOFF, nleaves + 1, &uoff));
+    PK_HIP(c, hipMemcpyAsync(lsd, ls.data(), nleaves * 8, hipMemcpyHostToDevice, st));
+    PK_HIP(c, hipMemcpyAsync(lud, lus.data(), nleaves * 8, hipMemcpyHostToDevice, st));
+    lstart = lsd;
+    lu = lud;
+  }
+  PK_HIP(c, hipEventRecord(c->ev[9], st));
+  PK_TRY(scan_excl<long long>(c, lu, uoff, nleaves));
+  ull U = 0;
+  PK_TRY(d2h(c, &U, uoff + nleaves, 8));
+  c->U = U;
+  PK_TRY(need(c, B_TK, U, &TK));
+  PK_TRY(need(c, B_TC, U, &TC));
+  PK_HIP(c, hipEventRecord(c->ev[10], st));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_compact<Key>), dim3((unsigned)nleaves), dim3(256), 0, st, S,
+                     C, lstart, lu, uoff, nleaves, TK, TC);
+  PK_HIP(c, hipGetLastError());
+  PK_HIP(c, hipEventRecord(c->ev[11], st));
+  PK_HIP(c, hipStreamSynchronize(st));
+  auto el = [&](int a, int b) {
+    float v = 0;
+    (void)hipEventElapsedTime(&v, c->ev[a], c->ev[b]);
+    return v;
+  };
+  c->timing.parse_ms = el(0, 1);
+  c->timing.hist_ms = el(2, 3);
+  c->timing.scatter_ms = el(4, 5);
+  c->timing.sort_ms = el(6, 7);
+  c->timing.split_ms = novf ? el(8, 9) : 0.f;
+  c->timing.compact_ms = el(10, 11);
+  c->have_table = true;
+  return PK_OK;
+}
+
+template <class Key>
+int cutoff_impl(pk_ctx* c, int t, uint64_t* hist_out, int* mb_out, uint64_t* nk_out) {
+  hipStream_t st = c->stream;
+  PK_HIP(c, hipEventRecord(c->ev[12], st));
+  Key* TK = reinterpret_cast<Key*>(c->buf[B_TK].p);
+  int32_t* TC = reinterpret_cast<int32_t*>(c->buf[B_TC].p);
+  const uint64_t U = c->U;
+  ull* hist;
+  ull* small;
+  PK_TRY(need(c, B_HIST, (uint64_t)t + 8, &hist));
+  PK_TRY(need(c, B_SMALL, 16, &small));
+  PK_TRY(fill_u64(c, hist, 0, (uint64_t)t));
+  const uint64_t nb = (U + SCAN_PER - 1) / SCAN_PER;
+  if (nb) {
+    hipLaunchKernelGGL(k_count_hist, dim3((unsigned)nb), dim3(CUT_NT), 0, st, TC, U, t, hist);
+    PK_HIP(c, hipGetLastError());
+  }
+  int* mbd = reinterpret_cast<int*>(small + 8);
+  hipLaunchKernelGGL(k_min_bucket, dim3(1), dim3(1), 0, st, hist, t, mbd);
+  PK_HIP(c, hipGetLastError());
+  ull *bs, *boffs;
+  PK_TRY(need(c, B_KBS, nb + 1, &bs));
+  PK_TRY(need(c, B_KBOFF, nb + 1, &boffs));
+  if (nb) {
+    hipLaunchKernelGGL(k_kept_count, dim3((unsigned)nb), dim3(CUT_NT), 0, st, TC, U, mbd, bs);
+    PK_HIP(c, hipGetLastError());
+  }
+  PK_TRY(scan_excl<ull>(c, bs, boffs, nb));
+  ull V = 0;
+  int mb = 0;
+  PK_HIP(c, hipMemcpyAsync(&V, boffs + nb, 8, hipMemcpyDeviceToHost, st));
+  PK_HIP(c, hipMemcpyAsync(&mb, mbd, 4, hipMemcpyDeviceToHost, st));
+  if (hist_out) PK_HIP(c, hipMemcpyAsync(hist_out, hist, (size_t)t * 8, hipMemcpyDeviceToHost, st));
+  PK_HIP(c, hipStreamSynchronize(st));
+  Key* KK;
+  int32_t* KC;
+  PK_TRY(need(c, B_KK, V, &KK));
+  PK_TRY(need(c, B_KC2, V, &KC));
+  if (nb && V) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_kept_write<Key>), dim3((unsigned)nb), dim3(CUT_NT), 0, st,
+                       TK, TC, U, mbd, boffs, KK, KC);
+    PK_HIP(c, hipGetLastError());
+  }
+  PK_HIP(c, hipEventRecord(c->ev[13], st));
+  PK_HIP(c, hipStreamSynchronize(st));
+  float ms = 0;
+  PK_HIP(c, hipEventElapsedTime(&ms, c->ev[12], c->ev[13]));
+  c->timing.cutoff_ms = ms;
+  c->V = V;
+  c->t = t;
+  c->mb = mb;
+  c->have_kept = true;
+  c->have_nodes = false;
+  *mb_out = mb;
+  *nk_out = V;
+  if (V == 0)
+    return c->fail(PK_EEMPTY, "no k-mer has count >= min_bucket(" + std::to_string(mb) + ")");
+  return PK_OK;
+}
+
+template <class Key>
+int mn_impl(pk_ctx* c) {
+  hipStream_t st = c->stream;
+  PK_HIP(c, hipEventRecord(c->ev[14], st));
+  const Key* KK = reinterpret_cast<const Key*>(c->buf[B_KK].p);
+  const int32_t* KC = reinterpret_cast<const int32_t*>(c->buf[B_KC2].p);
+  const uint64_t V = c->V;
+  const int K = c->k;
+  const uint64_t G = (V + MN_PC - 1) / MN_PC;
+  ull *q, *small, *ng, *noff;
+  MnPlan* plan;
+  MnRec *NT, *nodes;
+  uint32_t* gflags;
+  PK_TRY(need(c, B_MNQ, 8, &q));
+  PK_TRY(need(c, B_SMALL, 16, &small));
+  PK_TRY(need(c, B_MNPLAN, G, &plan));
+  PK_TRY(need(c, B_MNNT, 2 * V, &NT));
+  PK_TRY(need(c, B_MNNG, G, &ng));
+  PK_TRY(need(c, B_MNNOFF, G + 1, &noff));
+  PK_TRY(need(c, B_MNFLAG, 2 * V, &gflags));
+  ull* maxe = small + 10;
+  PK_TRY(fill_u64(c, maxe, 0, 1));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mn_quarters<Key>), dim3(1), dim3(64), 0, st, KK, V, K, q);
+  PK_HIP(c, hipGetLastError());
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mn_plan<Key>), dim3(grid_for(G, 256)), dim3(256), 0, st, KK,
+                     V, K, q, G, plan, maxe);
+  PK_HIP(c, hipGetLastError());
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mn_build<Key, false>), dim3((unsigned)G), dim3(MN_NT), 0, st,
+                     KK, KC, K, plan, G, NT, ng, gflags);
+  PK_HIP(c, hipGetLastError());
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mn_build<Key, true>), dim3((unsigned)G), dim3(MN_NT), 0, st,
+                     KK, KC, K, plan, G, NT, ng, gflags);
+  PK_HIP(c, hipGetLastError());
+  PK_TRY(scan_excl<ull>(c, ng, noff, G));
+  ull M = 0;
+  PK_TRY(d2h(c, &M, noff + G, 8));
+  PK_TRY(need(c, B_NODES, M, &nodes));
+  hipLaunchKernelGGL(k_mn_compact, dim3((unsigned)G), dim3(256), 0, st, NT, plan, ng, noff, G, nodes);
+  PK_HIP(c, hipGetLastError());
+  PK_HIP(c, hipEventRecord(c->ev[15], st));
+  PK_HIP(c, hipStreamSynchronize(st));
+  float ms = 0;
+  PK_HIP(c, hipEventElapsedTime(&ms, c->ev[14], c->ev[15]));
+  c->timing.mn_ms = ms;
+  c->M = M;
+  c->have_nodes = true;
+  return PK_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI.
+
+extern "C" {
+
+int pk_ctx_create(pk_ctx** out, int device, uint64_t hbm_budget_bytes) {
+  if (!out) return PK_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PK_EHIP;
+  if (device < 0 || device >= ndev) return PK_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return PK_EHIP;
+  pk_ctx* c = new pk_ctx();
+  c->device = device;
+  c->budget = hbm_budget_bytes;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return PK_EHIP;
+  }
+  for (auto& e : c->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete c;
+      return PK_EHIP;
+    }
+  }
+  *out = c;
+  return PK_OK;
+}
+
+void pk_ctx_destroy(pk_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& b : c->buf)
+    if (b.p) (void)hipFree(b.p);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* pk_last_error(const pk_ctx* c) { return c ? c->err.c_str() : "null context"; }
+void* pk_ctx_stream(pk_ctx* c) { return c ? (void*)c->stream : nullptr; }
+uint64_t pk_ctx_device_bytes(const pk_ctx* c) { return c ? c->held : 0; }
+
+int pk_count(pk_ctx* c, const char* fasta, uint64_t nbytes, int k, int fasta_on_device,
+             pk_table* out) {
+  if (!c || !out || (!fasta && nbytes)) return PK_EINVAL;
+  if (k < 8 || k > 64) return c->fail(PK_EINVAL, "k must be in [8, 64]");
+  PK_HIP(c, hipSetDevice(c->device));
+  const uint8_t* fa;
+  if (fasta_on_device) {
+    fa = reinterpret_cast<const uint8_t*>(fasta);
+  } else {
+    uint8_t* d;
+    PK_TRY(need(c, B_FASTA, nbytes + 64, &d));
+    if (nbytes) PK_HIP(c, hipMemcpyAsync(d, fasta, nbytes, hipMemcpyHostToDevice, c->stream));
+    fa = d;
+  }
+  int rc = (k <= 32) ? count_impl<uint64_t>(c, fa, nbytes, k) : count_impl<u128>(c, fa, nbytes, k);
+  if (rc) return rc;
+  out->n = c->U;
+  out->words = c->words;
+  out->k = c->k;
+  out->keys = reinterpret_cast<const uint64_t*>(c->buf[B_TK].p);
+  out->counts = reinterpret_cast<const int32_t*>(c->buf[B_TC].p);
+  return PK_OK;
+}
+
+int pk_cutoff(pk_ctx* c, int t, uint64_t* hist_t, int* min_bucket, uint64_t* n_kept) {
+  if (!c || !min_bucket || !n_kept) return PK_EINVAL;
+  if (!c->have_table) return c->fail(PK_ESTATE, "pk_cutoff before pk_count");
+  if (t == 0) t = 21;
+  if (t < 0 || t > T_MAX) return c->fail(PK_EINVAL, "t must be in [0, 4096]");
+  PK_HIP(c, hipSetDevice(c->device));
+  return c->words == 1 ? cutoff_impl<uint64_t>(c, t, hist_t, min_bucket, n_kept)
+                       : cutoff_impl<u128>(c, t, hist_t, min_bucket, n_kept);
+}
+
+int pk_kept_table(pk_ctx* c, pk_table* out) {
+  if (!c || !out) return PK_EINVAL;
+  if (!c->have_kept) return c->fail(PK_ESTATE, "no kept table");
+  out->n = c->V;
+  out->words = c->words;
+  out->k = c->k;
+  out->keys = reinterpret_cast<const uint64_t*>(c->buf[B_KK].p);
+  out->counts = reinterpret_cast<const int32_t*>(c->buf[B_KC2].p);
+  return PK_OK;
+}
+
+int pk_macro_nodes(pk_ctx* c, pk_mn_table* out) {
+  if (!c || !out) return PK_EINVAL;
+  if (!c->have_kept || c->V == 0) return c->fail(PK_ESTATE, "pk_macro_nodes needs a non-empty kept table");
+  PK_HIP(c, hipSetDevice(c->device));
+  int rc = c->words == 1 ? mn_impl<uint64_t>(c) : mn_impl<u128>(c);
+  if (rc) return rc;
+  out->n = c->M;
+  out->k = c->k;
+  out->nodes = reinterpret_cast<const pk_mnode*>(c->buf[B_NODES].p);
+  return PK_OK;
+}
+
+int pk_download_table(pk_ctx* c, uint64_t* keys, int32_t* counts) {
+  if (!c) return PK_EINVAL;
+  if (!c->have_table) return c->fail(PK_ESTATE, "no table");
+  if (c->U == 0) return PK_OK;
+  if (keys) PK_TRY(d2h(c, keys, c->buf[B_TK].p, c->U * c->words * 8));
+  if (counts) PK_TRY(d2h(c, counts, c->buf[B_TC].p, c->U * 4));
+  return PK_OK;
+}
+
+int pk_download_kept(pk_ctx* c, uint64_t* keys, int32_t* counts) {
+  if (!c) return PK_EINVAL;
+  if (!c->have_kept) return c->fail(PK_ESTATE, "no kept table");
+  if (c->V == 0) return PK_OK;
+  if (keys) PK_TRY(d2h(c, keys, c->buf[B_KK].p, c->V * c->words * 8));
+  if (counts) PK_TRY(d2h(c, counts, c->buf[B_KC2].p, c->V * 4));
+  return PK_OK;
+}
+
+int pk_download_nodes(pk_ctx* c, pk_mnode* nodes) {
+  if (!c || !nodes) return PK_EINVAL;
+  if (!c->have_nodes) return c->fail(PK_ESTATE, "no macro nodes");
+  if (c->M == 0) return PK_OK;
+  return d2h(c, nodes, c->buf[B_NODES].p, c->M * sizeof(pk_mnode));
+}
+
+int pk_last_timing(const pk_ctx* c, pk_timing* out) {
+  if (!c || !out) return PK_EINVAL;
+  *out = c->timing;
+  return PK_OK;
+}
+
+}  // extern "C"

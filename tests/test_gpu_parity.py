@@ -1,0 +1,210 @@
+"""HIP library (through its C ABI) against the CPU oracle and the reference's
+own cutoff / macro-node output.  Bit-exact: keys, counts, histogram,
+min_bucket, kept table and macro-node records."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from golden_util import SYNTH, read_fasta, read_ref, ref_nodes_as_arrays
+
+pytestmark = pytest.mark.gpu
+
+
+def _p(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from pakman_plus_amd import api
+    c = api.Context(0)
+    yield c
+    c.close()
+
+
+def _words(k):
+    return 1 if k <= 32 else 2
+
+
+def _check_table(ctx, oracle_mod, fa, k):
+    ctx.count(fa, k)
+    keys, counts = ctx.table_host()
+    ok, oc = oracle_mod.count(fa, k)
+    assert keys.shape == ok.shape, (keys.shape, ok.shape)
+    assert np.array_equal(keys, ok)
+    assert np.array_equal(counts, oc)
+    return keys, counts
+
+
+def _check_nodes(ctx, oracle_mod, k, kept_keys, kept_counts):
+    from pakman_plus_amd import api
+    m = ctx.macro_nodes()
+    got = api.nodes_as_arrays(ctx.nodes_host(), k)
+    want = oracle_mod.macro_nodes(kept_keys, kept_counts, k)
+    assert m == want["keys"].shape[0]
+    for f in ("keys", "pre_cnt", "suf_cnt"):
+        assert np.array_equal(got[f], want[f]), f
+    return got
+
+
+def test_reference_unit_test_input(ctx, oracle_mod):
+    _p("case: ref_test_two_reads k=32")
+    from pakman_plus_amd import api
+    fa = read_fasta("ref_test_two_reads.fa")
+    keys, counts = _check_table(ctx, oracle_mod, fa, 32)
+    assert int(counts.sum()) == 138  # src/concurrent_kmer_counter_test.cpp:96
+    with pytest.raises(api.PakHipError) as e:
+        ctx.cutoff(21)
+    assert e.value.code == api.PK_EEMPTY and ctx.min_bucket == 2
+
+
+@pytest.mark.parametrize("name,k,cov,t", [(n, k, c, t) for n, (c, ks, t) in SYNTH.items() for k in ks])
+def test_golden_pipeline(ctx, oracle_mod, name, k, cov, t):
+    _p(f"case: golden {name} k={k}")
+    fa = read_fasta(f"syn_{name}.fa.gz")
+    ref = read_ref(f"syn_{name}_k{k}.ref.txt.gz")
+    keys, counts = _check_table(ctx, oracle_mod, fa, k)
+    assert keys.shape[0] == ref.distinct
+    hist, mb, nk = ctx.cutoff(t)
+    ohist, omb, okeep, _ = oracle_mod.cutoff(counts, t)
+    assert np.array_equal(hist, ohist) and mb == omb == ref.min_bucket and nk == ref.kept
+    kk, kc = ctx.kept_host()
+    assert np.array_equal(kk, keys[okeep]) and np.array_equal(kc, counts[okeep])
+    got = _check_nodes(ctx, oracle_mod, k, kk, kc)
+    want = ref_nodes_as_arrays(ref, _words(k))
+    for f in ("keys", "pre_cnt", "suf_cnt"):
+        assert np.array_equal(got[f], want[f]), f
+
+
+@pytest.mark.parametrize("k", [8, 9, 15, 16, 17, 24, 31, 32, 33, 40, 48, 63, 64])
+def test_count_all_k(ctx, oracle_mod, k):
+    _p(f"case: tiny all-k k={k}")
+    fa = read_fasta("syn_tiny.fa.gz")
+    _check_table(ctx, oracle_mod, fa, k)
+
+
+def test_format_errors(ctx, oracle_mod):
+    _p("case: format errors")
+    from pakman_plus_amd import api
+    with pytest.raises(api.PakHipError) as e:
+        ctx.count(read_fasta("read_test1_raw.fa"), 32)
+    assert e.value.code == api.PK_EFORMAT
+    with pytest.raises(api.PakHipError) as e:
+        ctx.count(b">a\n" + b"A" * 40 + b"\n" + b"C" * 40 + b"\n>b\n" + b"G" * 40 + b"\n", 32)
+    assert e.value.code == api.PK_EFORMAT
+    with pytest.raises(api.PakHipError) as e:
+        ctx.count(b">a\n" + b"ACGT" * 10 + b"\nxx", 32)
+    assert e.value.code == api.PK_EFORMAT
+    with pytest.raises(api.PakHipError) as e:
+        ctx.count(b">a\nACGT\n", 7)
+    assert e.value.code == api.PK_EINVAL
+    # a tail too short to hold a k-mer is never inspected
+    _check_table(ctx, oracle_mod, b">a\n" + b"ACGT" * 10 + b"\n>b\nx\nyy", 32)
+    # the fixed READ_TEST1 parses
+    _check_table(ctx, oracle_mod, read_fasta("read_test1.fa"), 32)
+
+
+@pytest.mark.parametrize("fa", [
+    b"",
+    b"no header at all\n",
+    b">only header",
+    b">h\nACGT\n",
+    b"junk before\n>h\n" + b"ACGTTGCA" * 8 + b"\n",
+    b">h\n" + b"ACGTNNacgtRYKM" * 9 + b"\n>h2\n" + b"nNnNACGT" * 7,
+    b">h\r\n" + b"ACGT" * 20 + b"\n",
+])
+def test_edge_inputs(ctx, oracle_mod, fa):
+    _p(f"case: edge input len={len(fa)}")
+    for k in (8, 32, 33, 64):
+        _check_table(ctx, oracle_mod, fa, k)
+
+
+def test_skewed_buckets(ctx, oracle_mod):
+    """Identical and near-identical k-mers overflow a bucket's capacity and
+    go through the split path."""
+    _p("case: skewed buckets")
+    rng = np.random.default_rng(5)
+    parts = []
+    for i in range(3000):
+        parts.append(b">a%d\n" % i + b"A" * 100 + b"\n")
+    bases = np.frombuffer(b"ACGT", dtype=np.uint8)
+    for i in range(2000):
+        s = bases[rng.integers(0, 4, 100)].tobytes()
+        parts.append(b">r%d\n" % i + s + b"\n")
+    # low-complexity: (AC)^n with sparse substitutions -> many near-identical keys
+    for i in range(4000):
+        s = bytearray(b"AC" * 75)
+        for j in rng.integers(0, 150, 2):
+            s[j] = b"ACGT"[rng.integers(0, 4)]
+        parts.append(b">l%d\n" % i + bytes(s) + b"\n")
+    fa = b"".join(parts)
+    for k in (32, 64, 21):
+        _check_table(ctx, oracle_mod, fa, k)
+
+
+def test_device_resident_input(ctx, oracle_mod):
+    _p("case: device-resident input")
+    import torch
+    fa = read_fasta("syn_mid.fa.gz")
+    t = torch.from_numpy(np.frombuffer(fa, dtype=np.uint8).copy()).to("cuda:0")
+    torch.cuda.synchronize()
+    ctx.count(None, 32, device_ptr=t.data_ptr(), nbytes=t.numel())
+    keys, counts = ctx.table_host()
+    ok, oc = oracle_mod.count(fa, 32)
+    assert np.array_equal(keys, ok) and np.array_equal(counts, oc)
+
+
+def test_small_config_full(ctx, oracle_mod, gen_mod):
+    """1 Mbp genome, 100 bp reads, 30x, k=32, -t 21: table, cutoff and nodes."""
+    _p("case: small config 1 Mbp 30x")
+    fa = gen_mod.make_fasta(gen_mod.config_small())
+    keys, counts = _check_table(ctx, oracle_mod, fa, 32)
+    hist, mb, nk = ctx.cutoff(21)
+    ohist, omb, okeep, _ = oracle_mod.cutoff(counts, 21)
+    assert np.array_equal(hist, ohist) and mb == omb and nk == int(okeep.sum())
+    kk, kc = ctx.kept_host()
+    _check_nodes(ctx, oracle_mod, 32, kk, kc)
+
+
+def test_repeat_family_k64(ctx, oracle_mod, gen_mod):
+    """Scaled-down large-with-repeats workload at k=64 (two-word keys)."""
+    _p("case: repeats k=64 (2 Mbp)")
+    fa = gen_mod.make_fasta(gen_mod.config_large(scale=0.02))
+    keys, counts = _check_table(ctx, oracle_mod, fa, 64)
+    hist, mb, nk = ctx.cutoff(21)
+    ohist, omb, okeep, _ = oracle_mod.cutoff(counts, 21)
+    assert np.array_equal(hist, ohist) and mb == omb and nk == int(okeep.sum())
+    kk, kc = ctx.kept_host()
+    _check_nodes(ctx, oracle_mod, 64, kk, kc)
+
+
+def test_bacterial_full_size_properties(ctx, gen_mod):
+    """4.6 Mbp / 80x / k=32 at full size: size-independent properties (sum of
+    counts = number of k-mer instances, strictly ascending keys, counts >= 1,
+    kept table = filter of the table, node count bounds)."""
+    _p("case: bacterial full size properties")
+    spec = gen_mod.config_bacterial()
+    fa = gen_mod.make_fasta(spec)
+    R = spec.num_reads()
+    ctx.count(fa, 32)
+    keys, counts = ctx.table_host()
+    assert int(counts.astype(np.int64).sum()) == R * (spec.read_len - 32 + 1)
+    assert np.all(keys[1:, 0] > keys[:-1, 0]) and counts.min() >= 1
+    hist, mb, nk = ctx.cutoff(21)
+    h = np.bincount(counts[counts < 21], minlength=21)
+    assert np.array_equal(hist, h.astype(np.uint64))
+    assert mb == 1 + int(np.argmin(h[1:]))
+    kk, kc = ctx.kept_host()
+    sel = counts >= mb
+    assert np.array_equal(kk, keys[sel]) and np.array_equal(kc, counts[sel])
+    m = ctx.macro_nodes()
+    nodes = ctx.nodes_host()
+    assert nk <= m <= 2 * nk
+    # every kept k-mer appears exactly once as a suffix and once as a prefix
+    assert int((nodes["suf_cnt"] > 0).sum()) == nk and int((nodes["pre_cnt"] > 0).sum()) == nk
+    assert int(nodes["suf_cnt"].astype(np.int64).sum()) == int(kc.astype(np.int64).sum())
+    assert np.all(nodes["key_lo"][1:] > nodes["key_lo"][:-1])
