@@ -8,22 +8,23 @@
 //                     j0+2r+1, with the record rules and the format check of
 //                     the reference's Sliding_window (src/kmer_counting.cpp:
 //                     2890-2945); k-mer offsets by exclusive scan.
-//   3. partition      k_bucket_hist + k_bucket_scatter: every read is parsed
-//                     by one thread with the reference's rolling 3-bit update
-//                     (kmer_shift, distribute_kmers.h:798-805); the k-mer goes
-//                     to bucket = its top b1 bits.  Per-block LDS histograms,
-//                     one global atomic per (block, bucket) to reserve space.
+//   3. partition      multisplit by the top bits of the k-mer, <= 8 bits per
+//                     level.  Level 1 (k_l1_hist + k_l1_scatter) is fused with
+//                     the parse: every read is walked by one thread with the
+//                     reference's rolling 3-bit update (kmer_shift,
+//                     distribute_kmers.h:798-805); tiles of k-mers are sorted
+//                     by digit in LDS and written as contiguous runs.  Further
+//                     levels (k_ms_hist + k_ms_scatter) split each bucket.
 //                     This is the on-device stand-in for the reference's
 //                     owner routing + MPI_Alltoallv (kmer_counting.cpp:254-352).
-//   4. bucket count   k_bucket_sort: one 1024-thread workgroup per bucket holds
-//                     the bucket's keys in registers, counting-sorts them into
-//                     LDS by the next 12 bits in rounds of at most RC keys,
-//                     finishes each sub-bin (insertion sort, or a bitonic sort
-//                     of the round when a sub-bin is large) and run-length
-//                     encodes (the reference's sort + RLE + merge + aggregate,
-//                     kmer_counting.cpp:3057-3086, 169-250).  Buckets larger
-//                     than the register capacity are split further by
-//                     k_part_hist / k_part_scatter (host-driven levels).
+//   4. bucket count   k_bucket_hash: one 512-thread workgroup per bucket counts
+//                     the bucket's distinct keys in an LDS hash table, then
+//                     counting-sorts them by the next 12 bits and writes the
+//                     ascending (key, count) pairs (the reference's sort + RLE
+//                     + merge + aggregate, kmer_counting.cpp:3057-3086,
+//                     169-250).  Buckets with more distinct keys than a table
+//                     holds are split further by k_part_hist / k_part_scatter
+//                     (host-driven levels).
 //   5. compaction     k_compact: bucket results -> dense sorted table.
 // pk_cutoff: k_count_hist (LDS histogram of counts < t), k_min_bucket, ordered
 //   stream compaction (kmer_histogram.cpp:84-140).
@@ -140,6 +141,25 @@ __device__ __forceinline__ ull block_max(ull v) {
   return s;
 }
 
+template <int NT>
+__device__ __forceinline__ ull block_min(ull v) {
+  constexpr int NW = NT / 64;
+  __shared__ ull ws[NW];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    ull y = __shfl_xor(v, o, 64);
+    v = y < v ? y : v;
+  }
+  if (lane == 0) ws[w] = v;
+  __syncthreads();
+  ull s = ~0ull;
+#pragma unroll
+  for (int i = 0; i < NW; i++) s = ws[i] < s ? ws[i] : s;
+  __syncthreads();
+  return s;
+}
+
 // ---------------------------------------------------------------------------
 // Small utility kernels.
 
@@ -208,7 +228,7 @@ __device__ __forceinline__ void load16(const uint8_t* fa, uint64_t n, uint64_t b
 }
 
 __global__ __launch_bounds__(NL_NT) void k_nl_count(const uint8_t* fa, uint64_t n, ull* blk_cnt,
-                                                    ull* first_gt) {
+                                                    ull* blk_gt) {
   const uint64_t base = (uint64_t)blockIdx.x * NL_PER_BLOCK + (uint64_t)threadIdx.x * NL_PER_THREAD;
   uint8_t b[16];
   load16(fa, n, base, b);
@@ -220,15 +240,20 @@ __global__ __launch_bounds__(NL_NT) void k_nl_count(const uint8_t* fa, uint64_t 
       if (b[i] == '>' && gt == ~0ull) gt = base + i;
     }
   }
-  // wave min of the first '>' then one atomic per wave
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    ull y = __shfl_xor(gt, o, 64);
-    gt = y < gt ? y : gt;
-  }
-  if ((threadIdx.x & 63) == 0 && gt != ~0ull) atomicMin(first_gt, gt);
   c = block_sum<NL_NT>(c);
-  if (threadIdx.x == 0) blk_cnt[blockIdx.x] = c;
+  gt = block_min<NL_NT>(gt);
+  if (threadIdx.x == 0) {
+    blk_cnt[blockIdx.x] = c;
+    blk_gt[blockIdx.x] = gt;
+  }
+}
+
+// out[0] = min(in[0..n)) (one block).
+__global__ __launch_bounds__(1024) void k_min_reduce(const ull* in, uint64_t n, ull* out) {
+  ull m = ~0ull;
+  for (uint64_t i = threadIdx.x; i < n; i += 1024) m = in[i] < m ? in[i] : m;
+  m = block_min<1024>(m);
+  if (threadIdx.x == 0) *out = m;
 }
 
 __global__ __launch_bounds__(NL_NT) void k_nl_write(const uint8_t* fa, uint64_t n, const ull* blk_off,
@@ -336,124 +361,401 @@ __device__ __forceinline__ void for_each_kmer(const uint8_t* fa, uint64_t n, uin
 }
 
 // ---------------------------------------------------------------------------
-// Partition into buckets by the top b1 bits.
+// Partition (multisplit) into buckets by the top bits of the key, at most 8
+// bits per level so that every tile holds tens of keys per digit: each tile
+// is sorted by digit in LDS and written out as contiguous runs.
+//   level 1  k_l1_hist / k_l1_scatter: fused with the parse of the reads
+//   level 2+ k_ms_hist / k_ms_scatter: from the previous level's array
 
-constexpr int PART_NT = 512;
-constexpr int B1_MAX = 14;
+constexpr int L1_NT = 256;
+constexpr int MS_NT = 256;
 
 template <class Key>
-__global__ __launch_bounds__(PART_NT) void k_bucket_hist(const uint8_t* fa, uint64_t n,
-                                                         const ull* rstart, const uint32_t* rlen,
-                                                         uint64_t R, int K, Key mask, int b1,
-                                                         int reads_per_block, ull* gcnt) {
-  __shared__ uint32_t h[1 << B1_MAX];
-  const int nb = 1 << b1;
-  for (int i = threadIdx.x; i < nb; i += PART_NT) h[i] = 0;
+struct TileCfg {
+  static constexpr int Q = sizeof(Key) == 8 ? 16 : 8;  // keys per thread per tile
+  static constexpr int TQ = L1_NT * Q;                 // 4096 (u64) / 2048 (u128): 32 KB
+};
+
+// Level-1 histogram: digit = bits [2K - d1, 2K) of each k-mer.
+template <class Key>
+__global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n, const ull* rstart,
+                                                   const uint32_t* rlen, uint64_t R, int K, Key mask,
+                                                   int d1, int rpb, ull* gcnt) {
+  __shared__ uint32_t h[256];
+  const int nb = 1 << d1;
+  for (int i = threadIdx.x; i < nb; i += L1_NT) h[i] = 0;
   __syncthreads();
-  const uint64_t r0 = (uint64_t)blockIdx.x * reads_per_block;
-  uint64_t r1 = r0 + reads_per_block;
-  if (r1 > R) r1 = R;
-  const int bpos = 2 * K - b1;
-  for (uint64_t r = r0 + threadIdx.x; r < r1; r += PART_NT) {
+  const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
+  const uint64_t r1 = r0 + rpb < R ? r0 + rpb : R;
+  const int bpos = 2 * K - d1;
+  for (uint64_t r = r0 + threadIdx.x; r < r1; r += L1_NT)
     for_each_kmer<Key>(fa, n, rstart[r], rlen[r], K, mask,
-                       [&](Key km) { atomicAdd(&h[key_digit<Key>(km, bpos, b1)], 1u); });
-  }
+                       [&](Key km) { atomicAdd(&h[key_digit<Key>(km, bpos, d1)], 1u); });
   __syncthreads();
-  for (int i = threadIdx.x; i < nb; i += PART_NT)
+  for (int i = threadIdx.x; i < nb; i += L1_NT)
     if (h[i]) atomicAdd(&gcnt[i], (ull)h[i]);
 }
 
+// Sort the tile T0 (per-thread slices of np keys at tid*Q) by digit into T1,
+// reserve each digit's run in the output (global atomics on cur) and write
+// the runs out contiguously: out[boff[d] + reserved + rank].
+template <class Key, int NT, int Q>
+__device__ __forceinline__ void tile_split_write(Key* T0, Key* T1, int np, int dpos, int dbits,
+                                                 uint32_t* h, uint32_t* toff, uint32_t* tcur,
+                                                 ull* gb, const ull* boff, ull* cur, Key* out) {
+  const int nb = 1 << dbits;
+  for (int i = threadIdx.x; i < nb; i += NT) h[i] = 0;
+  __syncthreads();
+  for (int j = 0; j < np; j++) atomicAdd(&h[key_digit<Key>(T0[threadIdx.x * Q + j], dpos, dbits)], 1u);
+  __syncthreads();
+  // exclusive scan over nb <= 256 digits (one per thread for NT == 256)
+  {
+    const uint32_t v = (int)threadIdx.x < nb ? h[threadIdx.x] : 0u;
+    const ull ex = block_excl_scan<NT>(v);
+    if ((int)threadIdx.x < nb) {
+      toff[threadIdx.x] = (uint32_t)ex;
+      tcur[threadIdx.x] = (uint32_t)ex;
+      gb[threadIdx.x] = v ? atomicAdd(&cur[threadIdx.x], (ull)v) : 0ull;
+    }
+    if ((int)threadIdx.x == nb - 1) toff[nb] = (uint32_t)(ex + v);
+  }
+  __syncthreads();
+  for (int j = 0; j < np; j++) {
+    const Key k = T0[threadIdx.x * Q + j];
+    const uint32_t d = key_digit<Key>(k, dpos, dbits);
+    T1[atomicAdd(&tcur[d], 1u)] = k;
+  }
+  __syncthreads();
+  const int total = (int)toff[nb];
+  for (int p = threadIdx.x; p < total; p += NT) {
+    // digit of sorted position p: last d with toff[d] <= p
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if ((int)toff[mid] <= p) lo = mid; else hi = mid - 1;
+    }
+    out[boff[lo] + gb[lo] + (ull)(p - (int)toff[lo])] = T1[p];
+  }
+  __syncthreads();
+}
+
+// Level-1 scatter fused with the parse.  Every thread walks its reads
+// (r0 + tid, r0 + tid + L1_NT, ...) with the rolling update and emits up to Q
+// k-mers per tile into its LDS slice; the block then splits the tile.
 template <class Key>
-__global__ __launch_bounds__(PART_NT) void k_bucket_scatter(const uint8_t* fa, uint64_t n,
-                                                            const ull* rstart, const uint32_t* rlen,
-                                                            uint64_t R, int K, Key mask, int b1,
-                                                            int reads_per_block, const ull* boff,
-                                                            ull* bcur, Key* S) {
-  __shared__ uint32_t h[1 << B1_MAX];
-  const int nb = 1 << b1;
-  for (int i = threadIdx.x; i < nb; i += PART_NT) h[i] = 0;
-  __syncthreads();
-  const uint64_t r0 = (uint64_t)blockIdx.x * reads_per_block;
-  uint64_t r1 = r0 + reads_per_block;
-  if (r1 > R) r1 = R;
-  const int bpos = 2 * K - b1;
-  for (uint64_t r = r0 + threadIdx.x; r < r1; r += PART_NT) {
-    for_each_kmer<Key>(fa, n, rstart[r], rlen[r], K, mask,
-                       [&](Key km) { atomicAdd(&h[key_digit<Key>(km, bpos, b1)], 1u); });
+__global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_t n,
+                                                      const ull* rstart, const uint32_t* rlen,
+                                                      uint64_t R, int K, Key mask, int d1, int rpb,
+                                                      const ull* boff, ull* cur, Key* S1) {
+  constexpr int Q = TileCfg<Key>::Q;
+  constexpr int TQ = TileCfg<Key>::TQ;
+  __shared__ Key T0[TQ];
+  __shared__ Key T1[TQ];
+  __shared__ uint32_t h[256], toff[257], tcur[256];
+  __shared__ ull gb[256];
+  const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
+  const uint64_t r1 = r0 + rpb < R ? r0 + rpb : R;
+  const int bpos = 2 * K - d1;
+  const bool aligned = (reinterpret_cast<uintptr_t>(fa) & 7) == 0;
+  uint64_t r = r0 + threadIdx.x;
+  uint64_t s = 0;
+  uint32_t len = 0, i = 0;
+  if (r < r1) {
+    s = rstart[r];
+    len = rlen[r];
   }
-  __syncthreads();
-  // reserve this block's range in every bucket (offset relative to the bucket)
-  for (int i = threadIdx.x; i < nb; i += PART_NT)
-    if (h[i]) h[i] = (uint32_t)atomicAdd(&bcur[i], (ull)h[i]);
-  __syncthreads();
-  for (uint64_t r = r0 + threadIdx.x; r < r1; r += PART_NT) {
-    for_each_kmer<Key>(fa, n, rstart[r], rlen[r], K, mask, [&](Key km) {
-      const uint32_t b = key_digit<Key>(km, bpos, b1);
-      const uint32_t rel = atomicAdd(&h[b], 1u);
-      S[boff[b] + rel] = km;
-    });
+  Key km = 0;
+  uint64_t wa = ~0ull, w = 0;
+  while (__syncthreads_or(r < r1)) {
+    int np = 0;
+    while (np < Q && r < r1) {
+      if (i >= len) {
+        r += L1_NT;
+        i = 0;
+        km = 0;
+        if (r < r1) {
+          s = rstart[r];
+          len = rlen[r];
+        }
+        continue;
+      }
+      const uint64_t p = s + i;
+      uint32_t c;
+      if (aligned) {
+        const uint64_t a = p & ~7ull;
+        if (a != wa) {
+          wa = a;
+          w = load_word8(fa, n, a);
+        }
+        c = (uint32_t)(w >> ((p & 7) * 8)) & 0xFFu;
+      } else {
+        c = fa[p];
+      }
+      km = ((km << 2) | (Key)((c >> 1) & 7u)) & mask;
+      i++;
+      if (i >= (uint32_t)K) T0[threadIdx.x * Q + np++] = km;
+    }
+    tile_split_write<Key, L1_NT, Q>(T0, T1, np, bpos, d1, h, toff, tcur, gb, boff, cur, S1);
   }
+}
+
+// tiles of a level: tp[a+1] = tp[a] + ceil(n_a / tile) for a < nb (one block)
+__global__ __launch_bounds__(1024) void k_tile_prefix(const ull* boff, int nb, int tile, ull* tp) {
+  __shared__ ull tmp[1025];
+  ull acc = 0;
+  if (threadIdx.x == 0) tp[0] = 0;
+  for (int base = 0; base < nb; base += 1024) {
+    const int a = base + threadIdx.x;
+    const ull v = a < nb ? (boff[a + 1] - boff[a] + tile - 1) / tile : 0ull;
+    const ull ex = block_excl_scan<1024>(v);
+    if (a < nb) tp[a + 1] = acc + ex + v;
+    tmp[threadIdx.x] = ex + v;
+    __syncthreads();
+    acc += tmp[1023];
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ int tile_owner(const ull* tp, int nb, ull blk) {
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tp[mid] <= blk) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Level >= 2 histogram: bucket a of the previous level (nb buckets, offsets
+// boff) is split by dbits more bits at dpos; counts go to gcnt[a << dbits | s].
+template <class Key>
+__global__ __launch_bounds__(MS_NT) void k_ms_hist(const Key* Sin, const ull* boff, const ull* tp, int nb,
+                                                   int dpos, int dbits, ull* gcnt) {
+  constexpr int TQ = TileCfg<Key>::TQ;
+  __shared__ uint32_t h[256];
+  const ull blk = blockIdx.x;
+  if (blk >= tp[nb]) return;
+  const int a = tile_owner(tp, nb, blk);
+  const ull b0 = boff[a] + (blk - tp[a]) * TQ;
+  const ull b1 = b0 + TQ < boff[a + 1] ? b0 + TQ : boff[a + 1];
+  const int ns = 1 << dbits;
+  for (int i = threadIdx.x; i < ns; i += MS_NT) h[i] = 0;
+  __syncthreads();
+  for (ull i = b0 + threadIdx.x; i < b1; i += MS_NT) atomicAdd(&h[key_digit<Key>(Sin[i], dpos, dbits)], 1u);
+  __syncthreads();
+  for (int i = threadIdx.x; i < ns; i += MS_NT)
+    if (h[i]) atomicAdd(&gcnt[((ull)a << dbits) + i], (ull)h[i]);
+}
+
+template <class Key>
+__global__ __launch_bounds__(MS_NT) void k_ms_scatter(const Key* Sin, const ull* boff, const ull* tp,
+                                                      int nb, int dpos, int dbits, const ull* boff2,
+                                                      ull* cur2, Key* Sout) {
+  constexpr int Q = TileCfg<Key>::Q;
+  constexpr int TQ = TileCfg<Key>::TQ;
+  __shared__ Key T0[TQ];
+  __shared__ Key T1[TQ];
+  __shared__ uint32_t h[256], toff[257], tcur[256];
+  __shared__ ull gb[256];
+  const ull blk = blockIdx.x;
+  if (blk >= tp[nb]) return;
+  const int a = tile_owner(tp, nb, blk);
+  const ull b0 = boff[a] + (blk - tp[a]) * TQ;
+  const ull b1 = b0 + TQ < boff[a + 1] ? b0 + TQ : boff[a + 1];
+  const int m = (int)(b1 - b0);
+  // thread t owns tile slots [t*Q, t*Q + Q); load them (coalesced by slot)
+  for (int j = threadIdx.x; j < TQ; j += MS_NT) T0[j] = j < m ? Sin[b0 + j] : Key(0);
+  int np = m - (int)threadIdx.x * Q;
+  np = np < 0 ? 0 : (np > Q ? Q : np);
+  __syncthreads();
+  const ull base = (ull)a << dbits;
+  tile_split_write<Key, MS_NT, Q>(T0, T1, np, dpos, dbits, h, toff, tcur, gb, boff2 + base,
+                                  cur2 + base, Sout);
 }
 
 // ---------------------------------------------------------------------------
-// Per-bucket count: registers -> LDS counting sort by a 12-bit sub-digit in
-// rounds, sub-bin sort, run-length encode.
+// Per-bucket count: LDS hash table of the bucket's distinct keys (64-bit CAS,
+// linear probing; duplicates cost one atomic add), then a counting sort of
+// the distinct keys by a 12-bit sub-digit into LDS, a per-sub-bin insertion
+// sort (sub-bins hold ~1-2 distinct keys) and a coalesced write of the
+// bucket's (key, count) pairs.  This is the reference's sort + run-length +
+// merge + aggregate (kmer_counting.cpp:3057-3086, 254-352, 169-250) for the
+// keys of one bucket.
 
-constexpr int BS_NT = 512;
-constexpr int SUB_MAX_BITS = 12;
+constexpr int BS_NT = 256;
+constexpr int SUB_MAX_BITS = 10;
 constexpr int SUB_BINS = 1 << SUB_MAX_BITS;
-constexpr int MAX_ROUNDS = 48;
-constexpr int SMALL_BIN = 512;
 
 template <class Key>
-struct BucketCfg;
+struct HashCfg;
 template <>
-struct BucketCfg<uint64_t> {
-  static constexpr int KPT = 40;    // keys per thread held in registers
-  static constexpr int RC = 8192;   // keys per LDS round (64 KB)
+struct HashCfg<uint64_t> {
+  static constexpr int T = 4096;      // table slots
+  static constexpr int LIMIT = 3072;  // distinct keys per task (load 0.75)
 };
 template <>
-struct BucketCfg<u128> {
-  static constexpr int KPT = 24;
-  static constexpr int RC = 4096;
+struct HashCfg<u128> {
+  static constexpr int T = 2048;
+  static constexpr int LIMIT = 1536;
 };
+// tasks with more instances than this are split without trying
+constexpr uint64_t TASK_MAX_INSTANCES = 1u << 18;
 
-template <class Key>
-__device__ void lds_bitonic(Key* L, int P) {
-  for (int k = 2; k <= P; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < (P >> 1); i += BS_NT) {
-        const int idx = 2 * j * (i / j) + (i % j);
-        const int ptn = idx + j;
-        const bool up = (idx & k) == 0;
-        Key a = L[idx], b = L[ptn];
-        if ((a > b) == up) {
-          L[idx] = b;
-          L[ptn] = a;
-        }
-      }
-      __syncthreads();
-    }
-  }
+__device__ __forceinline__ ull mix64(ull x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
 }
 
-// Tasks: [tstart[i], tstart[i] + tcount[i]) of S.  Writes the task's unique
-// (key, count) pairs at the front of its range (S in place, C alongside) and
-// tu[i] = number of pairs, or tu[i] = -1 when the task does not fit (then the
-// task index is appended to ovf_list).
 template <class Key>
-__global__ __launch_bounds__(BS_NT) void k_bucket_sort(Key* S, int32_t* C, const ull* tstart,
+struct HashTable;
+
+// u64 keys are stored as themselves; EMPTY is a key outside the task.
+template <>
+struct HashTable<uint64_t> {
+  static constexpr int T = HashCfg<uint64_t>::T;
+  ull* hk;
+  uint32_t* hc;
+  ull empty;
+  __device__ __forceinline__ void init(int tid, int nt) {
+    for (int i = tid; i < T; i += nt) {
+      hk[i] = empty;
+      hc[i] = 0;
+    }
+  }
+  // returns false when the table is full
+  __device__ __forceinline__ bool insert(uint64_t k, uint32_t* ndist, int limit, int* over) {
+    uint32_t slot = (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> 52) & (T - 1);
+    for (int probe = 0; probe < T; probe++) {
+      const ull prev = atomicCAS(&hk[slot], empty, (ull)k);
+      if (prev == empty) {
+        if ((int)atomicAdd(ndist, 1u) >= limit) *over = 1;
+        atomicAdd(&hc[slot], 1u);
+        return true;
+      }
+      if (prev == (ull)k) {
+        atomicAdd(&hc[slot], 1u);
+        return true;
+      }
+      slot = (slot + 1) & (T - 1);
+    }
+    return false;
+  }
+  __device__ __forceinline__ bool used(int i) const { return hc[i] != 0u; }
+  __device__ __forceinline__ uint64_t key(int i) const { return hk[i]; }
+};
+
+// u128 keys: claim by 64-bit fingerprint, full key stored beside it and
+// verified after all inserts (a fingerprint collision sends the task to the
+// split path, which separates the two keys).
+template <>
+struct HashTable<u128> {
+  static constexpr int T = HashCfg<u128>::T;
+  static constexpr ull EMPTY = ~0ull;
+  ull* hf;
+  u128* hkf;
+  uint32_t* hc;
+  ull empty;
+  __device__ __forceinline__ static ull fp(u128 k) {
+    return mix64((ull)k ^ mix64((ull)(k >> 64) + 0x632be59bd9b4e019ull));
+  }
+  __device__ __forceinline__ void init(int tid, int nt) {
+    for (int i = tid; i < T; i += nt) {
+      hf[i] = EMPTY;
+      hc[i] = 0;
+    }
+  }
+  __device__ __forceinline__ uint32_t home(ull f) const { return (uint32_t)(f >> 53) & (T - 1); }
+  __device__ __forceinline__ bool insert(u128 k, uint32_t* ndist, int limit, int* over) {
+    const ull f = fp(k);
+    if (f == EMPTY) {
+      *over = 1;
+      return false;
+    }
+    uint32_t slot = home(f);
+    for (int probe = 0; probe < T; probe++) {
+      const ull prev = atomicCAS(&hf[slot], EMPTY, f);
+      if (prev == EMPTY) {
+        hkf[slot] = k;
+        if ((int)atomicAdd(ndist, 1u) >= limit) *over = 1;
+        atomicAdd(&hc[slot], 1u);
+        return true;
+      }
+      if (prev == f) {
+        atomicAdd(&hc[slot], 1u);
+        return true;
+      }
+      slot = (slot + 1) & (T - 1);
+    }
+    return false;
+  }
+  // after a barrier: does key k sit in its fingerprint's slot?
+  __device__ __forceinline__ bool verify(u128 k) const {
+    const ull f = fp(k);
+    uint32_t slot = home(f);
+    for (int probe = 0; probe < T; probe++) {
+      if (hf[slot] == f) return hkf[slot] == k;
+      if (hf[slot] == EMPTY) return false;
+      slot = (slot + 1) & (T - 1);
+    }
+    return false;
+  }
+  __device__ __forceinline__ bool used(int i) const { return hc[i] != 0u; }
+  __device__ __forceinline__ u128 key(int i) const { return hkf[i]; }
+};
+
+template <class Key>
+struct HashLds;
+template <>
+struct HashLds<uint64_t> {
+  ull hk[HashCfg<uint64_t>::T];
+  uint32_t hc[HashCfg<uint64_t>::T];
+  __device__ __forceinline__ HashTable<uint64_t> table(ull empty) {
+    HashTable<uint64_t> t;
+    t.hk = hk;
+    t.hc = hc;
+    t.empty = empty;
+    return t;
+  }
+  __device__ __forceinline__ uint64_t* out_keys() { return reinterpret_cast<uint64_t*>(hk); }
+};
+template <>
+struct HashLds<u128> {
+  ull hf[HashCfg<u128>::T];
+  u128 hkf[HashCfg<u128>::T];
+  uint32_t hc[HashCfg<u128>::T];
+  __device__ __forceinline__ HashTable<u128> table(ull) {
+    HashTable<u128> t;
+    t.hf = hf;
+    t.hkf = hkf;
+    t.hc = hc;
+    t.empty = 0;
+    return t;
+  }
+  __device__ __forceinline__ u128* out_keys() { return hkf; }
+};
+
+// Tasks: [tstart[i], tstart[i] + tcount[i]) of S, whose keys share their top
+// `fixed` bits (fixed >= 1).  Writes the task's unique (key, count) pairs in
+// ascending order at the front of its range (S in place, C alongside) and
+// tu[i] = number of pairs, or tu[i] = -1 when the task has to be split (too
+// many distinct keys, or a u128 fingerprint collision); the task index is
+// then appended to ovf_list.
+template <class Key>
+__global__ __launch_bounds__(BS_NT) void k_bucket_hash(Key* S, int32_t* C, const ull* tstart,
                                                        const ull* tcount, uint64_t ntasks,
-                                                       int dpos, int dbits, long long* tu,
+                                                       int K, int dpos, int dbits, long long* tu,
                                                        ull* ovf_count, ull* ovf_list) {
-  constexpr int KPT = BucketCfg<Key>::KPT;
-  constexpr int RC = BucketCfg<Key>::RC;
-  __shared__ Key L[RC];
+  constexpr int T = HashCfg<Key>::T;
+  constexpr int LIMIT = HashCfg<Key>::LIMIT;
+  constexpr int PER = T / BS_NT;  // table slots per thread
+  __shared__ HashLds<Key> lds;
   __shared__ uint32_t hs[SUB_BINS];
   __shared__ uint32_t he[SUB_BINS];
-  __shared__ int r_lo[MAX_ROUNDS], r_hi[MAX_ROUNDS];
-  __shared__ int n_rounds, bad, use_bitonic;
-  __shared__ ull run_base;
+  __shared__ uint32_t ndist;
+  __shared__ int over;
 
   const uint64_t task = blockIdx.x;
   if (task >= ntasks) return;
@@ -463,40 +765,69 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_sort(Key* S, int32_t* C, const
     if (threadIdx.x == 0) tu[task] = 0;
     return;
   }
-  if (n > (uint64_t)KPT * BS_NT) {
+  if (n > TASK_MAX_INSTANCES) {
     if (threadIdx.x == 0) {
       tu[task] = -1;
       ovf_list[atomicAdd(ovf_count, 1ull)] = task;
     }
     return;
   }
+  // EMPTY for u64 tables: the task's first key with its top key bit flipped
+  // (all keys of the task share that bit, so this value is not in the task)
+  const Key k0 = S[start];
+  const ull empty = (ull)(k0 ^ (Key(1) << (2 * K - 1)));
+  HashTable<Key> ht = lds.table(empty);
+  ht.init(threadIdx.x, BS_NT);
+  if (threadIdx.x == 0) {
+    ndist = 0;
+    over = 0;
+  }
+  __syncthreads();
+  for (uint64_t i = threadIdx.x; i < n; i += BS_NT) {
+    if (*(volatile int*)&over) break;
+    if (!ht.insert(S[start + i], &ndist, LIMIT, &over)) over = 1;
+  }
+  __syncthreads();
+  if constexpr (sizeof(Key) == 16) {
+    if (!over) {
+      for (uint64_t i = threadIdx.x; i < n; i += BS_NT)
+        if (!ht.verify(S[start + i])) over = 1;
+    }
+    __syncthreads();
+  }
+  if (over) {
+    if (threadIdx.x == 0) {
+      tu[task] = -1;
+      ovf_list[atomicAdd(ovf_count, 1ull)] = task;
+    }
+    return;
+  }
+  // gather this thread's slots (contiguous range) into registers
   const int nbins = 1 << dbits;
-  // keys tid, tid + BS_NT, ... of the task; nv of them are valid
-  const int nv = (int)((n > threadIdx.x) ? (n - threadIdx.x - 1) / BS_NT + 1 : 0);
-  Key keys[KPT];
-  const Key* src = S + start + threadIdx.x;
-#pragma unroll
-  for (int i = 0; i < KPT; i++) keys[i] = i < nv ? src[i * BS_NT] : Key(0);
+  Key rk[PER];
+  uint32_t rc[PER];
   for (int i = threadIdx.x; i < nbins; i += BS_NT) hs[i] = 0;
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < KPT; i++)
-    if (i < nv) atomicAdd(&hs[key_digit<Key>(keys[i], dpos, dbits)], 1u);
+  for (int j = 0; j < PER; j++) {
+    const int slot = threadIdx.x * PER + j;
+    rc[j] = lds.hc[slot];
+    rk[j] = rc[j] ? ht.key(slot) : Key(0);
+    if (rc[j]) atomicAdd(&hs[key_digit<Key>(rk[j], dpos, dbits)], 1u);
+  }
   __syncthreads();
-  // exclusive scan of hs over nbins (BPT bins per thread) -> hs = starts, he = ends
+  // exclusive scan of the sub-bin histogram
   {
     constexpr int BPT = SUB_BINS / BS_NT;
     uint32_t v[BPT];
-    ull s = 0, mx = 0;
+    ull s = 0;
 #pragma unroll
     for (int i = 0; i < BPT; i++) {
       const int b = threadIdx.x * BPT + i;
       v[i] = b < nbins ? hs[b] : 0u;
       s += v[i];
-      mx = v[i] > mx ? v[i] : mx;
     }
     ull ex = block_excl_scan<BS_NT>(s);
-    mx = block_max<BS_NT>(mx);
 #pragma unroll
     for (int i = 0; i < BPT; i++) {
       const int b = threadIdx.x * BPT + i;
@@ -506,108 +837,34 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_sort(Key* S, int32_t* C, const
         he[b] = (uint32_t)ex;
       }
     }
-    if (threadIdx.x == 0) {
-      use_bitonic = mx > (ull)SMALL_BIN;
-      bad = mx > (ull)RC;
-    }
   }
   __syncthreads();
-  if (threadIdx.x == 0 && !bad) {
-    // greedy rounds over bins: each round holds <= RC keys
-    int nr = 0, d = 0;
-    uint32_t s = 0;
-    while (d < nbins) {
-      if (nr >= MAX_ROUNDS) {
-        bad = 1;
-        break;
-      }
-      // largest e >= d with he[e] - s <= RC (he is non-decreasing)
-      int lo = d, hi = nbins - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (he[mid] - s <= (uint32_t)RC) lo = mid; else hi = mid - 1;
-      }
-      r_lo[nr] = d;
-      r_hi[nr] = lo + 1;
-      nr++;
-      s = he[lo];
-      d = lo + 1;
-    }
-    n_rounds = nr;
-    run_base = 0;
-  }
-  __syncthreads();
-  if (bad) {
-    if (threadIdx.x == 0) {
-      tu[task] = -1;
-      ovf_list[atomicAdd(ovf_count, 1ull)] = task;
-    }
-    return;
-  }
-  const int nr = n_rounds;
-  for (int rd = 0; rd < nr; rd++) {
-    const int blo = r_lo[rd], bhi = r_hi[rd];
-    const uint32_t pbase = blo ? he[blo - 1] : 0u;
-    const int m = (int)(he[bhi - 1] - pbase);
-    if (m == 0) continue;  // uniform: every thread reads the same LDS values
+  // scatter distinct keys into the table arrays in sub-bin order
+  Key* ok = lds.out_keys();
+  uint32_t* oc = lds.hc;
 #pragma unroll
-    for (int i = 0; i < KPT; i++) {
-      if (i < nv) {
-        const int d = (int)key_digit<Key>(keys[i], dpos, dbits);
-        if (d >= blo && d < bhi) {
-          const uint32_t pos = atomicAdd(&hs[d], 1u) - pbase;
-          L[pos] = keys[i];
-        }
-      }
+  for (int j = 0; j < PER; j++) {
+    if (rc[j]) {
+      const uint32_t pos = atomicAdd(&hs[key_digit<Key>(rk[j], dpos, dbits)], 1u);
+      ok[pos] = rk[j];
+      oc[pos] = rc[j];
     }
-    __syncthreads();
-    if (use_bitonic) {
-      int P = 1;
-      while (P < m) P <<= 1;
-      for (int i = m + threadIdx.x; i < P; i += BS_NT) L[i] = ~Key(0);
-      __syncthreads();
-      lds_bitonic<Key>(L, P);
-    } else {
-      for (int d = blo + threadIdx.x; d < bhi; d += BS_NT) {
-        const int a = (int)((d ? he[d - 1] : 0u) - pbase);
-        const int e = (int)(he[d] - pbase);
-        for (int i = a + 1; i < e; i++) {
-          const Key v = L[i];
-          int j = i - 1;
-          while (j >= a && L[j] > v) {
-            L[j + 1] = L[j];
-            j--;
-          }
-          L[j + 1] = v;
-        }
-      }
-    }
-    __syncthreads();
-    // run-length encode L[0, m)
-    const int c = (m + BS_NT - 1) / BS_NT;
-    const int p0 = threadIdx.x * c;
-    int p1 = p0 + c;
-    if (p1 > m) p1 = m;
-    ull starts = 0;
-    for (int p = p0; p < p1; p++) starts += (p == 0 || L[p] != L[p - 1]);
-    ull ridx = block_excl_scan<BS_NT>(starts);
-    const ull rb = run_base;
-    for (int p = p0; p < p1; p++) {
-      if (p == 0 || L[p] != L[p - 1]) {
-        int q = p + 1;
-        while (q < m && L[q] == L[p]) q++;
-        const uint64_t o = start + rb + ridx;
-        S[o] = L[p];
-        C[o] = q - p;
-        ridx++;
-      }
-    }
-    const ull my_runs = starts;
-    const ull all = block_sum<BS_NT>(my_runs);
-    if (threadIdx.x == 0) run_base = rb + all;
-    __syncthreads();
   }
-  if (threadIdx.x == 0) tu[task] = (long long)run_base;
+  __syncthreads();
+  // each distinct key's place in the task's ascending output: its sub-bin's
+  // start plus its rank among the (1-2 on average) keys of that sub-bin
+  const uint32_t nd = ndist;
+  for (uint32_t i = threadIdx.x; i < nd; i += BS_NT) {
+    const Key v = ok[i];
+    const int d = (int)key_digit<Key>(v, dpos, dbits);
+    const int a = d ? (int)he[d - 1] : 0;
+    const int e = (int)he[d];
+    int rank = 0;
+    for (int j = a; j < e; j++) rank += ok[j] < v;
+    S[start + a + rank] = v;
+    C[start + a + rank] = (int32_t)oc[i];
+  }
+  if (threadIdx.x == 0) tu[task] = (long long)nd;
 }
 
 // All keys of the task are identical (no bits left to split on).
@@ -1043,7 +1300,7 @@ enum BufId {
   B_FASTA, B_BLK, B_BLKOFF, B_NL, B_RSTART, B_RLEN, B_KC, B_KOFS, B_SMALL, B_BCNT, B_BOFF, B_BCUR,
   B_S, B_C, B_TU, B_UOFF, B_TK, B_TC, B_HIST, B_KBS, B_KBOFF, B_KK, B_KC2, B_MNQ, B_MNPLAN,
   B_MNNT, B_MNNG, B_MNNOFF, B_MNFLAG, B_NODES, B_TMP, B_OVF, B_PT0, B_PT1, B_PT2, B_PT3, B_PT4,
-  B_PT5, B_PT6, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
+  B_PT5, B_PT6, B_BLKGT, B_S2, B_BCNT2, B_BOFF2, B_BCUR2, B_TP, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
 };
 
 }  // namespace
@@ -1160,10 +1417,14 @@ int d2h(pk_ctx* c, void* dst, const void* src, size_t bytes) {
   return PK_OK;
 }
 
-int choose_b1(uint64_t N, int K, int cap_keys) {
-  int b1 = 0;
-  while (b1 < B1_MAX && b1 < 2 * K && (N >> b1) > (uint64_t)(cap_keys / 2)) b1++;
-  return b1;
+// Bucket bits: at least 1 (the hash table's EMPTY value flips the top key
+// bit), and enough that a bucket holds about 1.5 tables of instances (buckets
+// with more distinct keys than a table holds are split afterwards).
+int choose_bits(uint64_t N, int K, int cap_keys) {
+  int b = 1;
+  const uint64_t target = (uint64_t)cap_keys * 3 / 2;
+  while (b < 24 && b < 2 * K && (N >> b) > target) b++;
+  return b;
 }
 
 // Sub-digit window of a task whose keys share their top `fixed` bits.
@@ -1190,7 +1451,7 @@ template <class Key>
 int resolve_overflow(pk_ctx* c, Key* S, int32_t* C, int K, int fixed_bits,
                      const std::vector<ull>& starts, const std::vector<ull>& counts,
                      std::vector<std::vector<Leaf>>& leaves) {
-  constexpr int CAP = BucketCfg<Key>::KPT * BS_NT;
+  constexpr int CAP = HashCfg<Key>::LIMIT;
   leaves.assign(starts.size(), {});
   const int rem = 2 * K - fixed_bits;
   if (rem <= 0) {
@@ -1283,8 +1544,8 @@ int resolve_overflow(pk_ctx* c, Key* S, int32_t* C, int K, int fixed_bits,
   PK_HIP(c, hipMemcpyAsync(ss, sub_start.data(), ns * 8, hipMemcpyHostToDevice, c->stream));
   PK_HIP(c, hipMemcpyAsync(sc, sub_count.data(), ns * 8, hipMemcpyHostToDevice, c->stream));
   PK_TRY(fill_u64(c, ovc, 0, 1));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bucket_sort<Key>), dim3((unsigned)ns), dim3(BS_NT), 0,
-                     c->stream, S, C, ss, sc, ns, dpos, dbits, su, ovc, ovl);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bucket_hash<Key>), dim3((unsigned)ns), dim3(BS_NT), 0,
+                     c->stream, S, C, ss, sc, ns, K, dpos, dbits, su, ovc, ovl);
   PK_HIP(c, hipGetLastError());
   std::vector<long long> hu(ns);
   PK_TRY(d2h(c, hu.data(), su, ns * 8));
@@ -1312,15 +1573,16 @@ int resolve_overflow(pk_ctx* c, Key* S, int32_t* C, int K, int fixed_bits,
 
 template <class Key>
 int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
-  constexpr int CAP = BucketCfg<Key>::KPT * BS_NT;
+  constexpr int CAP = HashCfg<Key>::LIMIT;
   hipStream_t st = c->stream;
   c->have_table = c->have_kept = c->have_nodes = false;
   c->timing = pk_timing{};
   PK_HIP(c, hipEventRecord(c->ev[0], st));
   // 1. line index
   const uint64_t nb = (n + NL_PER_BLOCK - 1) / NL_PER_BLOCK;
-  ull *blk, *blkoff, *small, *nl;
+  ull *blk, *blkoff, *small, *nl, *blkgt;
   PK_TRY(need(c, B_BLK, nb, &blk));
+  PK_TRY(need(c, B_BLKGT, nb + 1, &blkgt));
   PK_TRY(need(c, B_BLKOFF, nb + 1, &blkoff));
   PK_TRY(need(c, B_SMALL, 16, &small));
   // small[0] = first '>', [1] = first unprocessed, [2] = first format error,
@@ -1328,7 +1590,9 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
   PK_TRY(fill_u64(c, small, ~0ull, 4));
   PK_TRY(fill_u64(c, small + 4, 0, 1));
   if (nb) {
-    hipLaunchKernelGGL(k_nl_count, dim3((unsigned)nb), dim3(NL_NT), 0, st, fa, n, blk, small);
+    hipLaunchKernelGGL(k_nl_count, dim3((unsigned)nb), dim3(NL_NT), 0, st, fa, n, blk, blkgt);
+    PK_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(1024), 0, st, blkgt, nb, small);
     PK_HIP(c, hipGetLastError());
   }
   PK_TRY(scan_excl<ull>(c, blk, blkoff, nb));
@@ -1381,31 +1645,71 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
     c->have_table = true;
     return PK_OK;
   }
-  // 3. partition
+  // 3. partition: level 1 fused with the parse, further levels from memory,
+  // at most 8 bits per level, until a bucket holds ~1.5 hash tables' worth
+  // of instances.
   const Key mask = key_mask<Key>(2 * K);
-  const int b1 = choose_b1(N, K, CAP);
-  const uint64_t B = 1ull << b1;
+  const int btot = choose_bits(N, K, CAP);
+  const int d1 = btot < 8 ? btot : 8;
+  const uint64_t B1 = 1ull << d1;
   ull *bcnt, *boff, *bcur;
-  PK_TRY(need(c, B_BCNT, B, &bcnt));
-  PK_TRY(need(c, B_BOFF, B + 1, &boff));
-  PK_TRY(need(c, B_BCUR, B, &bcur));
+  Key* S2;
+  PK_TRY(need(c, B_BCNT, B1, &bcnt));
+  PK_TRY(need(c, B_BOFF, B1 + 1, &boff));
+  PK_TRY(need(c, B_BCUR, B1, &bcur));
   PK_TRY(need(c, B_S, N, &S));
+  PK_TRY(need(c, B_S2, btot > d1 ? N : 1, &S2));
   PK_TRY(need(c, B_C, N, &C));
-  PK_TRY(fill_u64(c, bcnt, 0, B));
-  PK_TRY(fill_u64(c, bcur, 0, B));
+  PK_TRY(fill_u64(c, bcnt, 0, B1));
+  PK_TRY(fill_u64(c, bcur, 0, B1));
   const int rpb = 2048;  // reads per block
   const unsigned pgrid = grid_for(R, rpb);
-  c->timing.bucket_bits = b1;
+  c->timing.bucket_bits = btot;
   PK_HIP(c, hipEventRecord(c->ev[2], st));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bucket_hist<Key>), dim3(pgrid), dim3(PART_NT), 0, st, fa, n,
-                     rstart, rlen, R, K, mask, b1, rpb, bcnt);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_hist<Key>), dim3(pgrid), dim3(L1_NT), 0, st, fa, n, rstart,
+                     rlen, R, K, mask, d1, rpb, bcnt);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[3], st));
-  PK_TRY(scan_excl<ull>(c, bcnt, boff, B));
+  PK_TRY(scan_excl<ull>(c, bcnt, boff, B1));
   PK_HIP(c, hipEventRecord(c->ev[4], st));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bucket_scatter<Key>), dim3(pgrid), dim3(PART_NT), 0, st,
-                     fa, n, rstart, rlen, R, K, mask, b1, rpb, boff, bcur, S);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key>), dim3(pgrid), dim3(L1_NT), 0, st, fa, n,
+                     rstart, rlen, R, K, mask, d1, rpb, boff, bcur, S);
   PK_HIP(c, hipGetLastError());
+  uint64_t B = B1;
+  int fixed = d1;
+  int set = 0;
+  Key* Sin = S;
+  Key* Sout = S2;
+  while (fixed < btot) {
+    const int db = btot - fixed < 8 ? btot - fixed : 8;
+    const uint64_t B2 = B << db;
+    ull *cnt2, *boff2, *cur2, *tp;
+    const int o = set ? 0 : 1;  // the other array set
+    PK_TRY(need(c, o ? B_BCNT2 : B_BCNT, B2, &cnt2));
+    PK_TRY(need(c, o ? B_BOFF2 : B_BOFF, B2 + 1, &boff2));
+    PK_TRY(need(c, o ? B_BCUR2 : B_BCUR, B2, &cur2));
+    PK_TRY(need(c, B_TP, B + 1, &tp));
+    PK_TRY(fill_u64(c, cnt2, 0, B2));
+    PK_TRY(fill_u64(c, cur2, 0, B2));
+    hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(1024), 0, st, boff, (int)B, TileCfg<Key>::TQ, tp);
+    PK_HIP(c, hipGetLastError());
+    const uint64_t tiles = N / TileCfg<Key>::TQ + B + 1;  // upper bound on tp[B]
+    const int dpos = 2 * K - fixed - db;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_hist<Key>), dim3((unsigned)tiles), dim3(MS_NT), 0, st, Sin,
+                       boff, tp, (int)B, dpos, db, cnt2);
+    PK_HIP(c, hipGetLastError());
+    PK_TRY(scan_excl<ull>(c, cnt2, boff2, B2));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_scatter<Key>), dim3((unsigned)tiles), dim3(MS_NT), 0, st,
+                       Sin, boff, tp, (int)B, dpos, db, boff2, cur2, Sout);
+    PK_HIP(c, hipGetLastError());
+    std::swap(Sin, Sout);
+    bcnt = cnt2;
+    boff = boff2;
+    B = B2;
+    fixed += db;
+    set = o;
+  }
+  S = Sin;
   PK_HIP(c, hipEventRecord(c->ev[5], st));
   // 4. per-bucket count
   long long* tu;
@@ -1414,10 +1718,10 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
   PK_TRY(need(c, B_OVF, B + 1, &ovl));
   PK_TRY(need(c, B_UOFF, B + 1, &uoff));
   int dpos, dbits;
-  sub_digit(K, b1, &dpos, &dbits);
+  sub_digit(K, btot, &dpos, &dbits);
   PK_HIP(c, hipEventRecord(c->ev[6], st));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bucket_sort<Key>), dim3((unsigned)B), dim3(BS_NT), 0, st, S,
-                     C, boff, bcnt, B, dpos, dbits, tu, small + 4, ovl);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bucket_hash<Key>), dim3((unsigned)B), dim3(BS_NT), 0, st, S,
+                     C, boff, bcnt, B, K, dpos, dbits, tu, small + 4, ovl);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[7], st));
   ull novf = 0;
@@ -1443,7 +1747,7 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
         ob.push_back(b);
       }
     std::vector<std::vector<Leaf>> rec;
-    PK_TRY(resolve_overflow<Key>(c, S, C, K, b1, os, oc, rec));
+    PK_TRY(resolve_overflow<Key>(c, S, C, K, btot, os, oc, rec));
     std::vector<ull> ls;
     std::vector<long long> lus;
     size_t j = 0;
