@@ -170,6 +170,12 @@ __global__ void k_fill_u64(ull* p, ull v, uint64_t n) {
 
 __global__ void k_copy_u64(ull* dst, const ull* src) { *dst = *src; }
 
+// out[i] = off[i + 1] - off[i]
+__global__ void k_diff(const ull* off, uint64_t n, ull* out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = off[i + 1] - off[i];
+}
+
 // ---------------------------------------------------------------------------
 // Exclusive scan (u32 or u64 in, u64 out, out[n] = total).  4096 elements
 // per 1024-thread block, recursive over block sums.
@@ -376,11 +382,13 @@ struct TileCfg {
   static constexpr int TQ = L1_NT * Q;                 // 4096 (u64) / 2048 (u128): 32 KB
 };
 
-// Level-1 histogram: digit = bits [2K - d1, 2K) of each k-mer.
+// Level-1 histogram: digit = bits [2K - d1, 2K) of each k-mer.  Block b's
+// count of digit d goes to H[d * nblk + b] (digit-major), so an exclusive
+// scan of H gives every block the output base of each of its digit runs.
 template <class Key>
 __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n, const ull* rstart,
                                                    const uint32_t* rlen, uint64_t R, int K, Key mask,
-                                                   int d1, int rpb, ull* gcnt) {
+                                                   int d1, int rpb, uint32_t* H) {
   __shared__ uint32_t h[256];
   const int nb = 1 << d1;
   for (int i = threadIdx.x; i < nb; i += L1_NT) h[i] = 0;
@@ -392,67 +400,67 @@ __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n
     for_each_kmer<Key>(fa, n, rstart[r], rlen[r], K, mask,
                        [&](Key km) { atomicAdd(&h[key_digit<Key>(km, bpos, d1)], 1u); });
   __syncthreads();
-  for (int i = threadIdx.x; i < nb; i += L1_NT)
-    if (h[i]) atomicAdd(&gcnt[i], (ull)h[i]);
+  for (int i = threadIdx.x; i < nb; i += L1_NT) H[(ull)i * gridDim.x + blockIdx.x] = h[i];
 }
 
-// Sort the tile T0 (per-thread slices of np keys at tid*Q) by digit into T1,
-// reserve each digit's run in the output (global atomics on cur) and write
-// the runs out contiguously: out[boff[d] + reserved + rank].
-template <class Key, int NT, int Q>
-__device__ __forceinline__ void tile_split_write(Key* T0, Key* T1, int np, int dpos, int dbits,
-                                                 uint32_t* h, uint32_t* toff, uint32_t* tcur,
-                                                 ull* gb, const ull* boff, ull* cur, Key* out) {
+// Split a tile by digit: LDS histogram and exclusive scan (toff), a
+// counting-sort scatter of the keys into T1 (digit of every sorted slot in
+// D1), then coalesced stores: sorted slot p of digit d goes to
+// out[base[d] - toff[d] + p].  Keys come from get(j), j < np.
+template <class Key, int NT>
+struct TileLds {
+  Key T1[TileCfg<Key>::TQ];
+  uint8_t D1[TileCfg<Key>::TQ];
+  uint32_t h[256];
+  uint32_t tcur[256];
+  ull base[256];
+  long long delta[256];
+};
+
+template <class Key, int NT, class Get>
+__device__ __forceinline__ void tile_split_write(TileLds<Key, NT>& L, int np, Get&& get, int dpos,
+                                                 int dbits, Key* out) {
   const int nb = 1 << dbits;
-  for (int i = threadIdx.x; i < nb; i += NT) h[i] = 0;
+  for (int i = threadIdx.x; i < nb; i += NT) L.h[i] = 0;
   __syncthreads();
-  for (int j = 0; j < np; j++) atomicAdd(&h[key_digit<Key>(T0[threadIdx.x * Q + j], dpos, dbits)], 1u);
+  for (int j = 0; j < np; j++) atomicAdd(&L.h[key_digit<Key>(get(j), dpos, dbits)], 1u);
   __syncthreads();
-  // exclusive scan over nb <= 256 digits (one per thread for NT == 256)
-  {
-    const uint32_t v = (int)threadIdx.x < nb ? h[threadIdx.x] : 0u;
-    const ull ex = block_excl_scan<NT>(v);
-    if ((int)threadIdx.x < nb) {
-      toff[threadIdx.x] = (uint32_t)ex;
-      tcur[threadIdx.x] = (uint32_t)ex;
-      gb[threadIdx.x] = v ? atomicAdd(&cur[threadIdx.x], (ull)v) : 0ull;
-    }
-    if ((int)threadIdx.x == nb - 1) toff[nb] = (uint32_t)(ex + v);
+  // nb <= 256 == NT: one digit per thread
+  const uint32_t v = (int)threadIdx.x < nb ? L.h[threadIdx.x] : 0u;
+  const ull ex = block_excl_scan<NT>(v);
+  if ((int)threadIdx.x < nb) {
+    L.tcur[threadIdx.x] = (uint32_t)ex;
+    L.delta[threadIdx.x] = (long long)L.base[threadIdx.x] - (long long)ex;
   }
-  __syncthreads();
+  const int total = (int)block_sum<NT>(v);
   for (int j = 0; j < np; j++) {
-    const Key k = T0[threadIdx.x * Q + j];
+    const Key k = get(j);
     const uint32_t d = key_digit<Key>(k, dpos, dbits);
-    T1[atomicAdd(&tcur[d], 1u)] = k;
+    const uint32_t pos = atomicAdd(&L.tcur[d], 1u);
+    L.T1[pos] = k;
+    L.D1[pos] = (uint8_t)d;
   }
   __syncthreads();
-  const int total = (int)toff[nb];
-  for (int p = threadIdx.x; p < total; p += NT) {
-    // digit of sorted position p: last d with toff[d] <= p
-    int lo = 0, hi = nb - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if ((int)toff[mid] <= p) lo = mid; else hi = mid - 1;
-    }
-    out[boff[lo] + gb[lo] + (ull)(p - (int)toff[lo])] = T1[p];
-  }
+  for (int p = threadIdx.x; p < total; p += NT) out[L.delta[L.D1[p]] + p] = L.T1[p];
+  if ((int)threadIdx.x < nb) L.base[threadIdx.x] += v;
   __syncthreads();
 }
 
 // Level-1 scatter fused with the parse.  Every thread walks its reads
 // (r0 + tid, r0 + tid + L1_NT, ...) with the rolling update and emits up to Q
-// k-mers per tile into its LDS slice; the block then splits the tile.
+// k-mers per tile into its LDS slice; the block then splits the tile.  The
+// block's digit bases come from the scanned histogram O (same layout as H).
 template <class Key>
 __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_t n,
                                                       const ull* rstart, const uint32_t* rlen,
                                                       uint64_t R, int K, Key mask, int d1, int rpb,
-                                                      const ull* boff, ull* cur, Key* S1) {
+                                                      const ull* O, Key* S1) {
   constexpr int Q = TileCfg<Key>::Q;
   constexpr int TQ = TileCfg<Key>::TQ;
   __shared__ Key T0[TQ];
-  __shared__ Key T1[TQ];
-  __shared__ uint32_t h[256], toff[257], tcur[256];
-  __shared__ ull gb[256];
+  __shared__ TileLds<Key, L1_NT> L;
+  const int nb = 1 << d1;
+  for (int i = threadIdx.x; i < nb; i += L1_NT) L.base[i] = O[(ull)i * gridDim.x + blockIdx.x];
   const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
   const uint64_t r1 = r0 + rpb < R ? r0 + rpb : R;
   const int bpos = 2 * K - d1;
@@ -495,8 +503,17 @@ __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_
       i++;
       if (i >= (uint32_t)K) T0[threadIdx.x * Q + np++] = km;
     }
-    tile_split_write<Key, L1_NT, Q>(T0, T1, np, bpos, d1, h, toff, tcur, gb, boff, cur, S1);
+    Key* slice = T0 + threadIdx.x * Q;
+    tile_split_write<Key, L1_NT>(L, np, [&](int j) { return slice[j]; }, bpos, d1, S1);
   }
+}
+
+// bucket offsets of a level from its scanned digit-major histogram:
+// boff[d] = O[d * stride_of(d)] ... for level 1 stride = nblk.
+__global__ void k_l1_boff(const ull* O, int nb, uint64_t nblk, ull total, ull* boff) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d < nb) boff[d] = O[(ull)d * nblk];
+  if (d == nb) boff[nb] = total;
 }
 
 // tiles of a level: tp[a+1] = tp[a] + ceil(n_a / tile) for a < nb (one block)
@@ -525,51 +542,96 @@ __device__ __forceinline__ int tile_owner(const ull* tp, int nb, ull blk) {
   return lo;
 }
 
-// Level >= 2 histogram: bucket a of the previous level (nb buckets, offsets
-// boff) is split by dbits more bits at dpos; counts go to gcnt[a << dbits | s].
+// Level >= 2: bucket a of the previous level (nb buckets, offsets boff,
+// tiles tp) is split by dbits more bits at dpos.  Tile t of bucket a writes
+// its count of sub-digit s to H[tp[a] << dbits + s * ntiles(a) + t]: in
+// (a, s, t) order, so the exclusive scan of H holds every tile's output base
+// per sub-digit and H's scan at t = 0 is the start of sub-bucket (a, s).
+__device__ __forceinline__ ull ms_index(const ull* tp, int a, int s, ull t, int dbits) {
+  return (tp[a] << dbits) + (ull)s * (tp[a + 1] - tp[a]) + t;
+}
+
 template <class Key>
 __global__ __launch_bounds__(MS_NT) void k_ms_hist(const Key* Sin, const ull* boff, const ull* tp, int nb,
-                                                   int dpos, int dbits, ull* gcnt) {
+                                                   int dpos, int dbits, uint32_t* H) {
   constexpr int TQ = TileCfg<Key>::TQ;
   __shared__ uint32_t h[256];
   const ull blk = blockIdx.x;
   if (blk >= tp[nb]) return;
   const int a = tile_owner(tp, nb, blk);
-  const ull b0 = boff[a] + (blk - tp[a]) * TQ;
+  const ull t = blk - tp[a];
+  const ull b0 = boff[a] + t * TQ;
   const ull b1 = b0 + TQ < boff[a + 1] ? b0 + TQ : boff[a + 1];
   const int ns = 1 << dbits;
   for (int i = threadIdx.x; i < ns; i += MS_NT) h[i] = 0;
   __syncthreads();
   for (ull i = b0 + threadIdx.x; i < b1; i += MS_NT) atomicAdd(&h[key_digit<Key>(Sin[i], dpos, dbits)], 1u);
   __syncthreads();
-  for (int i = threadIdx.x; i < ns; i += MS_NT)
-    if (h[i]) atomicAdd(&gcnt[((ull)a << dbits) + i], (ull)h[i]);
+  for (int i = threadIdx.x; i < ns; i += MS_NT) H[ms_index(tp, a, i, t, dbits)] = h[i];
+}
+
+// boff2[a << dbits | s] = O[ms_index(a, s, 0)] for every (a, s); boff2[end] = N
+__global__ void k_ms_boff(const ull* O, const ull* tp, int nb, int dbits, ull total, ull* boff2) {
+  const ull g = (ull)blockIdx.x * blockDim.x + threadIdx.x;
+  const ull n2 = (ull)nb << dbits;
+  if (g < n2) {
+    const int a = (int)(g >> dbits);
+    const int s = (int)(g & ((1u << dbits) - 1));
+    boff2[g] = (tp[a + 1] > tp[a]) ? O[ms_index(tp, a, s, 0, dbits)] : O[tp[a] << dbits];
+  }
+  if (g == n2) boff2[n2] = total;
 }
 
 template <class Key>
 __global__ __launch_bounds__(MS_NT) void k_ms_scatter(const Key* Sin, const ull* boff, const ull* tp,
-                                                      int nb, int dpos, int dbits, const ull* boff2,
-                                                      ull* cur2, Key* Sout) {
+                                                      int nb, int dpos, int dbits, const ull* O,
+                                                      Key* Sout) {
   constexpr int Q = TileCfg<Key>::Q;
   constexpr int TQ = TileCfg<Key>::TQ;
-  __shared__ Key T0[TQ];
-  __shared__ Key T1[TQ];
-  __shared__ uint32_t h[256], toff[257], tcur[256];
-  __shared__ ull gb[256];
+  __shared__ TileLds<Key, MS_NT> L;
   const ull blk = blockIdx.x;
   if (blk >= tp[nb]) return;
   const int a = tile_owner(tp, nb, blk);
-  const ull b0 = boff[a] + (blk - tp[a]) * TQ;
+  const ull t = blk - tp[a];
+  const ull b0 = boff[a] + t * TQ;
   const ull b1 = b0 + TQ < boff[a + 1] ? b0 + TQ : boff[a + 1];
   const int m = (int)(b1 - b0);
-  // thread t owns tile slots [t*Q, t*Q + Q); load them (coalesced by slot)
-  for (int j = threadIdx.x; j < TQ; j += MS_NT) T0[j] = j < m ? Sin[b0 + j] : Key(0);
-  int np = m - (int)threadIdx.x * Q;
-  np = np < 0 ? 0 : (np > Q ? Q : np);
+  const int ns = 1 << dbits;
+  for (int i = threadIdx.x; i < ns; i += MS_NT) {
+    L.base[i] = O[ms_index(tp, a, i, t, dbits)];
+    L.h[i] = 0;
+  }
+  // keys tid, tid + MS_NT, ... of the tile in registers (coalesced loads)
+  Key kr[Q];
+#pragma unroll
+  for (int j = 0; j < Q; j++) {
+    const int idx = (int)threadIdx.x + j * MS_NT;
+    kr[j] = idx < m ? Sin[b0 + idx] : Key(0);
+  }
+  const int np = m > (int)threadIdx.x ? (m - (int)threadIdx.x - 1) / MS_NT + 1 : 0;
   __syncthreads();
-  const ull base = (ull)a << dbits;
-  tile_split_write<Key, MS_NT, Q>(T0, T1, np, dpos, dbits, h, toff, tcur, gb, boff2 + base,
-                                  cur2 + base, Sout);
+#pragma unroll
+  for (int j = 0; j < Q; j++)
+    if (j < np) atomicAdd(&L.h[key_digit<Key>(kr[j], dpos, dbits)], 1u);
+  __syncthreads();
+  const uint32_t v = (int)threadIdx.x < ns ? L.h[threadIdx.x] : 0u;
+  const ull ex = block_excl_scan<MS_NT>(v);
+  if ((int)threadIdx.x < ns) {
+    L.tcur[threadIdx.x] = (uint32_t)ex;
+    L.delta[threadIdx.x] = (long long)L.base[threadIdx.x] - (long long)ex;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < Q; j++) {
+    if (j < np) {
+      const uint32_t d = key_digit<Key>(kr[j], dpos, dbits);
+      const uint32_t pos = atomicAdd(&L.tcur[d], 1u);
+      L.T1[pos] = kr[j];
+      L.D1[pos] = (uint8_t)d;
+    }
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < m; p += MS_NT) Sout[L.delta[L.D1[p]] + p] = L.T1[p];
 }
 
 // ---------------------------------------------------------------------------
@@ -1300,7 +1362,7 @@ enum BufId {
   B_FASTA, B_BLK, B_BLKOFF, B_NL, B_RSTART, B_RLEN, B_KC, B_KOFS, B_SMALL, B_BCNT, B_BOFF, B_BCUR,
   B_S, B_C, B_TU, B_UOFF, B_TK, B_TC, B_HIST, B_KBS, B_KBOFF, B_KK, B_KC2, B_MNQ, B_MNPLAN,
   B_MNNT, B_MNNG, B_MNNOFF, B_MNFLAG, B_NODES, B_TMP, B_OVF, B_PT0, B_PT1, B_PT2, B_PT3, B_PT4,
-  B_PT5, B_PT6, B_BLKGT, B_S2, B_BCNT2, B_BOFF2, B_BCUR2, B_TP, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
+  B_PT5, B_PT6, B_BLKGT, B_S2, B_BCNT2, B_BOFF2, B_BCUR2, B_TP, B_HIST1, B_OFF1, B_HIST2, B_OFF2, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
 };
 
 }  // namespace
@@ -1647,69 +1709,82 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
   }
   // 3. partition: level 1 fused with the parse, further levels from memory,
   // at most 8 bits per level, until a bucket holds ~1.5 hash tables' worth
-  // of instances.
+  // of instances.  Write bases come from scanned digit-major histograms.
   const Key mask = key_mask<Key>(2 * K);
   const int btot = choose_bits(N, K, CAP);
   const int d1 = btot < 8 ? btot : 8;
   const uint64_t B1 = 1ull << d1;
-  ull *bcnt, *boff, *bcur;
+  const int rpb = 2048;  // reads per block
+  const uint64_t nblk = (R + rpb - 1) / rpb;
+  ull *boff, *O;
+  uint32_t* H;
   Key* S2;
-  PK_TRY(need(c, B_BCNT, B1, &bcnt));
   PK_TRY(need(c, B_BOFF, B1 + 1, &boff));
-  PK_TRY(need(c, B_BCUR, B1, &bcur));
+  PK_TRY(need(c, B_HIST1, B1 * nblk, &H));
+  PK_TRY(need(c, B_OFF1, B1 * nblk + 1, &O));
   PK_TRY(need(c, B_S, N, &S));
   PK_TRY(need(c, B_S2, btot > d1 ? N : 1, &S2));
   PK_TRY(need(c, B_C, N, &C));
-  PK_TRY(fill_u64(c, bcnt, 0, B1));
-  PK_TRY(fill_u64(c, bcur, 0, B1));
-  const int rpb = 2048;  // reads per block
-  const unsigned pgrid = grid_for(R, rpb);
   c->timing.bucket_bits = btot;
   PK_HIP(c, hipEventRecord(c->ev[2], st));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_hist<Key>), dim3(pgrid), dim3(L1_NT), 0, st, fa, n, rstart,
-                     rlen, R, K, mask, d1, rpb, bcnt);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_hist<Key>), dim3((unsigned)nblk), dim3(L1_NT), 0, st, fa, n,
+                     rstart, rlen, R, K, mask, d1, rpb, H);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[3], st));
-  PK_TRY(scan_excl<ull>(c, bcnt, boff, B1));
+  PK_TRY(scan_excl<uint32_t>(c, H, O, B1 * nblk));
+  hipLaunchKernelGGL(k_l1_boff, dim3(grid_for(B1 + 1, 256)), dim3(256), 0, st, O, (int)B1, nblk, (ull)N,
+                     boff);
+  PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[4], st));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key>), dim3(pgrid), dim3(L1_NT), 0, st, fa, n,
-                     rstart, rlen, R, K, mask, d1, rpb, boff, bcur, S);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key>), dim3((unsigned)nblk), dim3(L1_NT), 0, st, fa,
+                     n, rstart, rlen, R, K, mask, d1, rpb, O, S);
   PK_HIP(c, hipGetLastError());
   uint64_t B = B1;
   int fixed = d1;
   int set = 0;
   Key* Sin = S;
   Key* Sout = S2;
+  constexpr int TQ = TileCfg<Key>::TQ;
   while (fixed < btot) {
     const int db = btot - fixed < 8 ? btot - fixed : 8;
     const uint64_t B2 = B << db;
-    ull *cnt2, *boff2, *cur2, *tp;
-    const int o = set ? 0 : 1;  // the other array set
-    PK_TRY(need(c, o ? B_BCNT2 : B_BCNT, B2, &cnt2));
+    const uint64_t tiles = N / TQ + B + 1;  // upper bound on tp[B]
+    ull *boff2, *tp, *O2;
+    uint32_t* H2;
+    const int o = set ? 0 : 1;  // the other offset array
     PK_TRY(need(c, o ? B_BOFF2 : B_BOFF, B2 + 1, &boff2));
-    PK_TRY(need(c, o ? B_BCUR2 : B_BCUR, B2, &cur2));
     PK_TRY(need(c, B_TP, B + 1, &tp));
-    PK_TRY(fill_u64(c, cnt2, 0, B2));
-    PK_TRY(fill_u64(c, cur2, 0, B2));
-    hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(1024), 0, st, boff, (int)B, TileCfg<Key>::TQ, tp);
+    // alternate histogram buffers per level: the previous level's kernels
+    // may still be reading theirs
+    PK_TRY(need(c, o ? B_HIST2 : B_HIST1, tiles << db, &H2));
+    PK_TRY(need(c, o ? B_OFF2 : B_OFF1, (tiles << db) + 1, &O2));
+    hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(1024), 0, st, boff, (int)B, TQ, tp);
     PK_HIP(c, hipGetLastError());
-    const uint64_t tiles = N / TileCfg<Key>::TQ + B + 1;  // upper bound on tp[B]
+    ull ntiles = 0;
+    PK_TRY(d2h(c, &ntiles, tp + B, 8));
     const int dpos = 2 * K - fixed - db;
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_hist<Key>), dim3((unsigned)tiles), dim3(MS_NT), 0, st, Sin,
-                       boff, tp, (int)B, dpos, db, cnt2);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_hist<Key>), dim3((unsigned)ntiles), dim3(MS_NT), 0, st, Sin,
+                       boff, tp, (int)B, dpos, db, H2);
     PK_HIP(c, hipGetLastError());
-    PK_TRY(scan_excl<ull>(c, cnt2, boff2, B2));
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_scatter<Key>), dim3((unsigned)tiles), dim3(MS_NT), 0, st,
-                       Sin, boff, tp, (int)B, dpos, db, boff2, cur2, Sout);
+    PK_TRY(scan_excl<uint32_t>(c, H2, O2, ntiles << db));
+    hipLaunchKernelGGL(k_ms_boff, dim3(grid_for(B2 + 1, 256)), dim3(256), 0, st, O2, tp, (int)B, db,
+                       (ull)N, boff2);
+    PK_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_scatter<Key>), dim3((unsigned)ntiles), dim3(MS_NT), 0, st,
+                       Sin, boff, tp, (int)B, dpos, db, O2, Sout);
     PK_HIP(c, hipGetLastError());
     std::swap(Sin, Sout);
-    bcnt = cnt2;
     boff = boff2;
     B = B2;
     fixed += db;
     set = o;
   }
   S = Sin;
+  // bucket sizes of the last level
+  ull* bcnt;
+  PK_TRY(need(c, B_BCNT, B, &bcnt));
+  hipLaunchKernelGGL(k_diff, dim3(grid_for(B, 256)), dim3(256), 0, st, boff, B, bcnt);
+  PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[5], st));
   // 4. per-bucket count
   long long* tu;
