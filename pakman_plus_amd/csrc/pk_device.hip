@@ -644,20 +644,22 @@ __global__ __launch_bounds__(MS_NT) void k_ms_scatter(const Key* Sin, const ull*
 // keys of one bucket.
 
 constexpr int BS_NT = 256;
-constexpr int SUB_MAX_BITS = 10;
+constexpr int SUB_MAX_BITS = 9;
 constexpr int SUB_BINS = 1 << SUB_MAX_BITS;
 
 template <class Key>
 struct HashCfg;
 template <>
 struct HashCfg<uint64_t> {
-  static constexpr int T = 4096;      // table slots
-  static constexpr int LIMIT = 3072;  // distinct keys per task (load 0.75)
+  static constexpr int T = 2048;      // table slots
+  static constexpr int LIMIT = 1536;  // distinct keys per task (load 0.75)
+  static constexpr int SUB = 9;       // sub-digit bits of the ordering pass
 };
 template <>
 struct HashCfg<u128> {
-  static constexpr int T = 2048;
-  static constexpr int LIMIT = 1536;
+  static constexpr int T = 1024;
+  static constexpr int LIMIT = 768;
+  static constexpr int SUB = 8;
 };
 // tasks with more instances than this are split without trying
 constexpr uint64_t TASK_MAX_INSTANCES = 1u << 18;
@@ -689,7 +691,7 @@ struct HashTable<uint64_t> {
   }
   // returns false when the table is full
   __device__ __forceinline__ bool insert(uint64_t k, uint32_t* ndist, int limit, int* over) {
-    uint32_t slot = (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> 52) & (T - 1);
+    uint32_t slot = (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> 40) & (T - 1);
     for (int probe = 0; probe < T; probe++) {
       const ull prev = atomicCAS(&hk[slot], empty, (ull)k);
       if (prev == empty) {
@@ -729,7 +731,7 @@ struct HashTable<u128> {
       hc[i] = 0;
     }
   }
-  __device__ __forceinline__ uint32_t home(ull f) const { return (uint32_t)(f >> 53) & (T - 1); }
+  __device__ __forceinline__ uint32_t home(ull f) const { return (uint32_t)(f >> 40) & (T - 1); }
   __device__ __forceinline__ bool insert(u128 k, uint32_t* ndist, int limit, int* over) {
     const ull f = fp(k);
     if (f == EMPTY) {
@@ -801,18 +803,22 @@ struct HashLds<u128> {
 
 // Tasks: [tstart[i], tstart[i] + tcount[i]) of S, whose keys share their top
 // `fixed` bits (fixed >= 1).  Writes the task's unique (key, count) pairs in
-// ascending order at the front of its range (S in place, C alongside) and
-// tu[i] = number of pairs, or tu[i] = -1 when the task has to be split (too
-// many distinct keys, or a u128 fingerprint collision); the task index is
-// then appended to ovf_list.
+// ascending order to R / C at the task's start and tu[i] = their number.  A
+// task with more distinct keys than a table holds is processed in 2^s passes,
+// pass p taking the keys whose next s bits equal p (s = 1, 2, ... up to
+// MAX_PASS_BITS); beyond that, or on a u128 fingerprint collision, tu[i] = -1
+// and the task index goes to ovf_list for the host-driven split.
+constexpr int MAX_PASS_BITS = 4;
+
 template <class Key>
-__global__ __launch_bounds__(BS_NT) void k_bucket_hash(Key* S, int32_t* C, const ull* tstart,
-                                                       const ull* tcount, uint64_t ntasks,
-                                                       int K, int dpos, int dbits, long long* tu,
-                                                       ull* ovf_count, ull* ovf_list) {
+__global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int32_t* C,
+                                                       const ull* tstart, const ull* tcount,
+                                                       uint64_t ntasks, int K, int fixed,
+                                                       long long* tu, ull* ovf_count, ull* ovf_list) {
   constexpr int T = HashCfg<Key>::T;
   constexpr int LIMIT = HashCfg<Key>::LIMIT;
   constexpr int PER = T / BS_NT;  // table slots per thread
+  constexpr int HB = sizeof(Key) == 8 ? 16 : 8;
   __shared__ HashLds<Key> lds;
   __shared__ uint32_t hs[SUB_BINS];
   __shared__ uint32_t he[SUB_BINS];
@@ -827,118 +833,162 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(Key* S, int32_t* C, const
     if (threadIdx.x == 0) tu[task] = 0;
     return;
   }
-  if (n > TASK_MAX_INSTANCES) {
-    if (threadIdx.x == 0) {
-      tu[task] = -1;
-      ovf_list[atomicAdd(ovf_count, 1ull)] = task;
-    }
-    return;
-  }
+  const int rem = 2 * K - fixed;
   // EMPTY for u64 tables: the task's first key with its top key bit flipped
   // (all keys of the task share that bit, so this value is not in the task)
   const Key k0 = S[start];
   const ull empty = (ull)(k0 ^ (Key(1) << (2 * K - 1)));
   HashTable<Key> ht = lds.table(empty);
-  ht.init(threadIdx.x, BS_NT);
-  if (threadIdx.x == 0) {
-    ndist = 0;
-    over = 0;
-  }
-  __syncthreads();
-  for (uint64_t i = threadIdx.x; i < n; i += BS_NT) {
-    if (*(volatile int*)&over) break;
-    if (!ht.insert(S[start + i], &ndist, LIMIT, &over)) over = 1;
-  }
-  __syncthreads();
-  if constexpr (sizeof(Key) == 16) {
-    if (!over) {
-      for (uint64_t i = threadIdx.x; i < n; i += BS_NT)
-        if (!ht.verify(S[start + i])) over = 1;
+  const int max_s = rem < MAX_PASS_BITS ? rem : MAX_PASS_BITS;
+  bool done = false;
+  ull out = 0;
+  // a first try with all keys is pointless when there are too many of them
+  int s = 0;
+  while (s < max_s && (n >> s) > (uint64_t)TASK_MAX_INSTANCES) s++;
+  for (; s <= max_s && !done; s++) {
+    out = 0;
+    bool failed = false;
+    for (int pass = 0; pass < (1 << s) && !failed; pass++) {
+      ht.init(threadIdx.x, BS_NT);
+      if (threadIdx.x == 0) {
+        ndist = 0;
+        over = 0;
+      }
+      __syncthreads();
+      // keys are loaded HB at a time into registers so that the global-load
+      // latency is paid once per batch, not once per insert
+      for (uint64_t b = 0; b < n; b += (uint64_t)BS_NT * HB) {
+        Key kb[HB];
+#pragma unroll
+        for (int j = 0; j < HB; j++) {
+          const uint64_t idx = b + threadIdx.x + (uint64_t)j * BS_NT;
+          kb[j] = idx < n ? S[start + idx] : Key(0);
+        }
+        if (*(volatile int*)&over) break;
+#pragma unroll
+        for (int j = 0; j < HB; j++) {
+          const uint64_t idx = b + threadIdx.x + (uint64_t)j * BS_NT;
+          if (idx < n && (int)key_digit<Key>(kb[j], rem - s, s) == pass &&
+              !ht.insert(kb[j], &ndist, LIMIT, &over))
+            over = 1;
+        }
+      }
+      __syncthreads();
+      if constexpr (sizeof(Key) == 16) {
+        if (!over) {
+          for (uint64_t b = 0; b < n; b += (uint64_t)BS_NT * HB) {
+            Key kb[HB];
+#pragma unroll
+            for (int j = 0; j < HB; j++) {
+              const uint64_t idx = b + threadIdx.x + (uint64_t)j * BS_NT;
+              kb[j] = idx < n ? S[start + idx] : Key(0);
+            }
+#pragma unroll
+            for (int j = 0; j < HB; j++) {
+              const uint64_t idx = b + threadIdx.x + (uint64_t)j * BS_NT;
+              if (idx < n && (int)key_digit<Key>(kb[j], rem - s, s) == pass && !ht.verify(kb[j]))
+                over = 2;
+            }
+          }
+        }
+        __syncthreads();
+      }
+      if (over) {
+        // too many distinct keys: retry with one more split bit; a
+        // fingerprint collision (over == 2) goes to the host split
+        failed = true;
+        if (over == 2) s = max_s;
+        __syncthreads();
+        break;
+      }
+      // order the distinct keys: counting sort by the sub-digit below the
+      // pass bits, then rank inside each sub-bin (1-2 keys on average)
+      const int r2 = rem - s;
+      const int dbits = r2 < SUB_MAX_BITS ? r2 : SUB_MAX_BITS;
+      const int dpos = r2 - dbits;
+      const int nbins = 1 << dbits;
+      Key rk[PER];
+      uint32_t rc[PER];
+      for (int i = threadIdx.x; i < nbins; i += BS_NT) hs[i] = 0;
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < PER; j++) {
+        const int slot = threadIdx.x * PER + j;
+        rc[j] = lds.hc[slot];
+        rk[j] = rc[j] ? ht.key(slot) : Key(0);
+        if (rc[j]) atomicAdd(&hs[key_digit<Key>(rk[j], dpos, dbits)], 1u);
+      }
+      __syncthreads();
+      {
+        constexpr int BPT = SUB_BINS / BS_NT;
+        uint32_t v[BPT];
+        ull sm = 0;
+#pragma unroll
+        for (int i = 0; i < BPT; i++) {
+          const int bb = threadIdx.x * BPT + i;
+          v[i] = bb < nbins ? hs[bb] : 0u;
+          sm += v[i];
+        }
+        ull ex = block_excl_scan<BS_NT>(sm);
+#pragma unroll
+        for (int i = 0; i < BPT; i++) {
+          const int bb = threadIdx.x * BPT + i;
+          if (bb < nbins) {
+            hs[bb] = (uint32_t)ex;
+            ex += v[i];
+            he[bb] = (uint32_t)ex;
+          }
+        }
+      }
+      __syncthreads();
+      Key* ok = lds.out_keys();
+      uint32_t* oc = lds.hc;
+#pragma unroll
+      for (int j = 0; j < PER; j++) {
+        if (rc[j]) {
+          const uint32_t pos = atomicAdd(&hs[key_digit<Key>(rk[j], dpos, dbits)], 1u);
+          ok[pos] = rk[j];
+          oc[pos] = rc[j];
+        }
+      }
+      __syncthreads();
+      const uint32_t nd = ndist;
+      for (uint32_t i = threadIdx.x; i < nd; i += BS_NT) {
+        const Key v = ok[i];
+        const int d = (int)key_digit<Key>(v, dpos, dbits);
+        const int a = d ? (int)he[d - 1] : 0;
+        const int e = (int)he[d];
+        int rank = 0;
+        for (int j = a; j < e; j++) rank += ok[j] < v;
+        R[start + out + a + rank] = v;
+        C[start + out + a + rank] = (int32_t)oc[i];
+      }
+      out += nd;
+      __syncthreads();
     }
-    __syncthreads();
+    if (!failed) done = true;
   }
-  if (over) {
-    if (threadIdx.x == 0) {
+  if (threadIdx.x == 0) {
+    if (done) {
+      tu[task] = (long long)out;
+    } else {
       tu[task] = -1;
       ovf_list[atomicAdd(ovf_count, 1ull)] = task;
     }
-    return;
   }
-  // gather this thread's slots (contiguous range) into registers
-  const int nbins = 1 << dbits;
-  Key rk[PER];
-  uint32_t rc[PER];
-  for (int i = threadIdx.x; i < nbins; i += BS_NT) hs[i] = 0;
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < PER; j++) {
-    const int slot = threadIdx.x * PER + j;
-    rc[j] = lds.hc[slot];
-    rk[j] = rc[j] ? ht.key(slot) : Key(0);
-    if (rc[j]) atomicAdd(&hs[key_digit<Key>(rk[j], dpos, dbits)], 1u);
-  }
-  __syncthreads();
-  // exclusive scan of the sub-bin histogram
-  {
-    constexpr int BPT = SUB_BINS / BS_NT;
-    uint32_t v[BPT];
-    ull s = 0;
-#pragma unroll
-    for (int i = 0; i < BPT; i++) {
-      const int b = threadIdx.x * BPT + i;
-      v[i] = b < nbins ? hs[b] : 0u;
-      s += v[i];
-    }
-    ull ex = block_excl_scan<BS_NT>(s);
-#pragma unroll
-    for (int i = 0; i < BPT; i++) {
-      const int b = threadIdx.x * BPT + i;
-      if (b < nbins) {
-        hs[b] = (uint32_t)ex;
-        ex += v[i];
-        he[b] = (uint32_t)ex;
-      }
-    }
-  }
-  __syncthreads();
-  // scatter distinct keys into the table arrays in sub-bin order
-  Key* ok = lds.out_keys();
-  uint32_t* oc = lds.hc;
-#pragma unroll
-  for (int j = 0; j < PER; j++) {
-    if (rc[j]) {
-      const uint32_t pos = atomicAdd(&hs[key_digit<Key>(rk[j], dpos, dbits)], 1u);
-      ok[pos] = rk[j];
-      oc[pos] = rc[j];
-    }
-  }
-  __syncthreads();
-  // each distinct key's place in the task's ascending output: its sub-bin's
-  // start plus its rank among the (1-2 on average) keys of that sub-bin
-  const uint32_t nd = ndist;
-  for (uint32_t i = threadIdx.x; i < nd; i += BS_NT) {
-    const Key v = ok[i];
-    const int d = (int)key_digit<Key>(v, dpos, dbits);
-    const int a = d ? (int)he[d - 1] : 0;
-    const int e = (int)he[d];
-    int rank = 0;
-    for (int j = a; j < e; j++) rank += ok[j] < v;
-    S[start + a + rank] = v;
-    C[start + a + rank] = (int32_t)oc[i];
-  }
-  if (threadIdx.x == 0) tu[task] = (long long)nd;
 }
 
 // All keys of the task are identical (no bits left to split on).
 template <class Key>
-__global__ void k_single_key(int32_t* C, const ull* tstart, const ull* tcount, uint64_t ntasks,
-                             long long* tu) {
+__global__ void k_single_key(const Key* S, Key* R, int32_t* C, const ull* tstart, const ull* tcount,
+                             uint64_t ntasks, long long* tu) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ntasks) return;
   if (tcount[i] == 0) {
     tu[i] = 0;
     return;
   }
+  R[tstart[i]] = S[tstart[i]];
   C[tstart[i]] = (int32_t)(uint32_t)tcount[i];
   tu[i] = 1;
 }
@@ -1480,27 +1530,15 @@ int d2h(pk_ctx* c, void* dst, const void* src, size_t bytes) {
 }
 
 // Bucket bits: at least 1 (the hash table's EMPTY value flips the top key
-// bit), and enough that a bucket holds about 1.5 tables of instances (buckets
-// with more distinct keys than a table holds are split afterwards).
+// bit), and enough that a bucket holds about 3 tables of instances (a bucket
+// with more distinct keys than a table holds takes extra passes).
 int choose_bits(uint64_t N, int K, int cap_keys) {
   int b = 1;
-  const uint64_t target = (uint64_t)cap_keys * 3 / 2;
+  const uint64_t target = (uint64_t)cap_keys * 3;
   while (b < 24 && b < 2 * K && (N >> b) > target) b++;
   return b;
 }
 
-// Sub-digit window of a task whose keys share their top `fixed` bits.
-void sub_digit(int K, int fixed, int* dpos, int* dbits) {
-  const int rem = 2 * K - fixed;
-  if (rem <= 0) {
-    *dpos = 0;
-    *dbits = 0;
-    return;
-  }
-  const int nb = rem < SUB_MAX_BITS ? rem : SUB_MAX_BITS;
-  *dpos = rem - nb;
-  *dbits = nb;
-}
 
 struct Leaf {
   ull start;
@@ -1510,7 +1548,7 @@ struct Leaf {
 // Recursively split oversize tasks; returns the leaves (in key order) for
 // each input task.
 template <class Key>
-int resolve_overflow(pk_ctx* c, Key* S, int32_t* C, int K, int fixed_bits,
+int resolve_overflow(pk_ctx* c, Key* S, Key* R, int32_t* C, int K, int fixed_bits,
                      const std::vector<ull>& starts, const std::vector<ull>& counts,
                      std::vector<std::vector<Leaf>>& leaves) {
   constexpr int CAP = HashCfg<Key>::LIMIT;
@@ -1527,7 +1565,7 @@ int resolve_overflow(pk_ctx* c, Key* S, int32_t* C, int K, int fixed_bits,
     PK_HIP(c, hipMemcpyAsync(ts, starts.data(), nt * 8, hipMemcpyHostToDevice, c->stream));
     PK_HIP(c, hipMemcpyAsync(tc, counts.data(), nt * 8, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_single_key<Key>), dim3(grid_for(nt, 256)), dim3(256), 0,
-                       c->stream, C, ts, tc, nt, tu);
+                       c->stream, S, R, C, ts, tc, nt, tu);
     PK_HIP(c, hipGetLastError());
     std::vector<long long> hu(nt);
     PK_TRY(d2h(c, hu.data(), tu, nt * 8));
@@ -1593,8 +1631,6 @@ int resolve_overflow(pk_ctx* c, Key* S, int32_t* C, int K, int fixed_bits,
   PK_HIP(c, hipGetLastError());
   // sort the sub-tasks that fit
   const int fixed2 = fixed_bits + pbits;
-  int dpos, dbits;
-  sub_digit(K, fixed2, &dpos, &dbits);
   const uint64_t ns = sub_start.size();
   ull *ss, *sc, *ovc, *ovl;
   long long* su;
@@ -1607,7 +1643,7 @@ int resolve_overflow(pk_ctx* c, Key* S, int32_t* C, int K, int fixed_bits,
   PK_HIP(c, hipMemcpyAsync(sc, sub_count.data(), ns * 8, hipMemcpyHostToDevice, c->stream));
   PK_TRY(fill_u64(c, ovc, 0, 1));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bucket_hash<Key>), dim3((unsigned)ns), dim3(BS_NT), 0,
-                     c->stream, S, C, ss, sc, ns, K, dpos, dbits, su, ovc, ovl);
+                     c->stream, S, R, C, ss, sc, ns, K, fixed2, su, ovc, ovl);
   PK_HIP(c, hipGetLastError());
   std::vector<long long> hu(ns);
   PK_TRY(d2h(c, hu.data(), su, ns * 8));
@@ -1622,7 +1658,7 @@ int resolve_overflow(pk_ctx* c, Key* S, int32_t* C, int K, int fixed_bits,
   std::vector<std::vector<Leaf>> sub_leaves(ns);
   if (!o_start.empty()) {
     std::vector<std::vector<Leaf>> rec;
-    PK_TRY(resolve_overflow<Key>(c, S, C, K, fixed2, o_start, o_count, rec));
+    PK_TRY(resolve_overflow<Key>(c, S, R, C, K, fixed2, o_start, o_count, rec));
     for (size_t j = 0; j < o_idx.size(); j++) sub_leaves[o_idx[j]] = rec[j];
   }
   for (size_t i = 0; i < ns; i++) {
@@ -1723,7 +1759,7 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
   PK_TRY(need(c, B_HIST1, B1 * nblk, &H));
   PK_TRY(need(c, B_OFF1, B1 * nblk + 1, &O));
   PK_TRY(need(c, B_S, N, &S));
-  PK_TRY(need(c, B_S2, btot > d1 ? N : 1, &S2));
+  PK_TRY(need(c, B_S2, N, &S2));
   PK_TRY(need(c, B_C, N, &C));
   c->timing.bucket_bits = btot;
   PK_HIP(c, hipEventRecord(c->ev[2], st));
@@ -1780,6 +1816,7 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
     set = o;
   }
   S = Sin;
+  Key* Res = Sout;  // the other array receives the (key, count) results
   // bucket sizes of the last level
   ull* bcnt;
   PK_TRY(need(c, B_BCNT, B, &bcnt));
@@ -1792,11 +1829,9 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
   PK_TRY(need(c, B_TU, B, &tu));
   PK_TRY(need(c, B_OVF, B + 1, &ovl));
   PK_TRY(need(c, B_UOFF, B + 1, &uoff));
-  int dpos, dbits;
-  sub_digit(K, btot, &dpos, &dbits);
   PK_HIP(c, hipEventRecord(c->ev[6], st));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bucket_hash<Key>), dim3((unsigned)B), dim3(BS_NT), 0, st, S,
-                     C, boff, bcnt, B, K, dpos, dbits, tu, small + 4, ovl);
+                     Res, C, boff, bcnt, B, K, btot, tu, small + 4, ovl);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[7], st));
   ull novf = 0;
@@ -1822,7 +1857,7 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
         ob.push_back(b);
       }
     std::vector<std::vector<Leaf>> rec;
-    PK_TRY(resolve_overflow<Key>(c, S, C, K, btot, os, oc, rec));
+    PK_TRY(resolve_overflow<Key>(c, S, Res, C, K, btot, os, oc, rec));
     std::vector<ull> ls;
     std::vector<long long> lus;
     size_t j = 0;
@@ -1857,7 +1892,7 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
   PK_TRY(need(c, B_TK, U, &TK));
   PK_TRY(need(c, B_TC, U, &TC));
   PK_HIP(c, hipEventRecord(c->ev[10], st));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_compact<Key>), dim3((unsigned)nleaves), dim3(256), 0, st, S,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_compact<Key>), dim3((unsigned)nleaves), dim3(256), 0, st, Res,
                      C, lstart, lu, uoff, nleaves, TK, TC);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[11], st));
