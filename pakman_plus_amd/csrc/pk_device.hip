@@ -447,8 +447,8 @@ __device__ __forceinline__ void tile_split_write(TileLds<Key, NT>& L, int np, Ge
 }
 
 // Level-1 scatter fused with the parse.  Every thread walks its reads
-// (r0 + tid, r0 + tid + L1_NT, ...) with the rolling update and emits up to Q
-// k-mers per tile into its LDS slice; the block then splits the tile.  The
+// (r0 + tid, r0 + tid + L1_NT, ...) with the rolling update and produces up
+// to Q k-mers per tile into registers; the block then splits the tile.  The
 // block's digit bases come from the scanned histogram O (same layout as H).
 template <class Key>
 __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_t n,
@@ -456,8 +456,6 @@ __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_
                                                       uint64_t R, int K, Key mask, int d1, int rpb,
                                                       const ull* O, Key* S1) {
   constexpr int Q = TileCfg<Key>::Q;
-  constexpr int TQ = TileCfg<Key>::TQ;
-  __shared__ Key T0[TQ];
   __shared__ TileLds<Key, L1_NT> L;
   const int nb = 1 << d1;
   for (int i = threadIdx.x; i < nb; i += L1_NT) L.base[i] = O[(ull)i * gridDim.x + blockIdx.x];
@@ -475,36 +473,73 @@ __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_
   Key km = 0;
   uint64_t wa = ~0ull, w = 0;
   while (__syncthreads_or(r < r1)) {
+    Key kr[Q];
     int np = 0;
-    while (np < Q && r < r1) {
-      if (i >= len) {
-        r += L1_NT;
-        i = 0;
-        km = 0;
-        if (r < r1) {
-          s = rstart[r];
-          len = rlen[r];
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+      kr[j] = Key(0);
+      bool got = false;
+      while (!got && r < r1) {
+        if (i >= len) {
+          r += L1_NT;
+          i = 0;
+          km = 0;
+          if (r < r1) {
+            s = rstart[r];
+            len = rlen[r];
+          }
+          continue;
         }
-        continue;
-      }
-      const uint64_t p = s + i;
-      uint32_t c;
-      if (aligned) {
-        const uint64_t a = p & ~7ull;
-        if (a != wa) {
-          wa = a;
-          w = load_word8(fa, n, a);
+        const uint64_t p = s + i;
+        uint32_t c;
+        if (aligned) {
+          const uint64_t a = p & ~7ull;
+          if (a != wa) {
+            wa = a;
+            w = load_word8(fa, n, a);
+          }
+          c = (uint32_t)(w >> ((p & 7) * 8)) & 0xFFu;
+        } else {
+          c = fa[p];
         }
-        c = (uint32_t)(w >> ((p & 7) * 8)) & 0xFFu;
-      } else {
-        c = fa[p];
+        km = ((km << 2) | (Key)((c >> 1) & 7u)) & mask;
+        i++;
+        if (i >= (uint32_t)K) {
+          kr[j] = km;
+          got = true;
+        }
       }
-      km = ((km << 2) | (Key)((c >> 1) & 7u)) & mask;
-      i++;
-      if (i >= (uint32_t)K) T0[threadIdx.x * Q + np++] = km;
+      np += got ? 1 : 0;
     }
-    Key* slice = T0 + threadIdx.x * Q;
-    tile_split_write<Key, L1_NT>(L, np, [&](int j) { return slice[j]; }, bpos, d1, S1);
+    // produced keys are kr[0..np) (a thread stops producing only at the end
+    // of its reads, so the filled slots are a prefix)
+    const int nbits = d1;
+    for (int q = threadIdx.x; q < (1 << nbits); q += L1_NT) L.h[q] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < Q; j++)
+      if (j < np) atomicAdd(&L.h[key_digit<Key>(kr[j], bpos, nbits)], 1u);
+    __syncthreads();
+    const uint32_t v = (int)threadIdx.x < nb ? L.h[threadIdx.x] : 0u;
+    const ull ex = block_excl_scan<L1_NT>(v);
+    if ((int)threadIdx.x < nb) {
+      L.tcur[threadIdx.x] = (uint32_t)ex;
+      L.delta[threadIdx.x] = (long long)L.base[threadIdx.x] - (long long)ex;
+    }
+    const int total = (int)block_sum<L1_NT>(v);
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+      if (j < np) {
+        const uint32_t d = key_digit<Key>(kr[j], bpos, nbits);
+        const uint32_t pos = atomicAdd(&L.tcur[d], 1u);
+        L.T1[pos] = kr[j];
+        L.D1[pos] = (uint8_t)d;
+      }
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < total; p += L1_NT) S1[L.delta[L.D1[p]] + p] = L.T1[p];
+    if ((int)threadIdx.x < nb) L.base[threadIdx.x] += v;
+    __syncthreads();
   }
 }
 
@@ -1088,6 +1123,57 @@ __global__ __launch_bounds__(CUT_NT) void k_count_hist(const int32_t* TC, uint64
     if (h[i]) atomicAdd(&hist[i], (ull)h[i]);
 }
 
+// Small-t path (t <= CUT_SMALL_T): per-wave LDS sub-histograms (no
+// contention between waves), the block's histogram stored to BH[b * t + c]
+// without global atomics; column sums give hist, and the kept count of block
+// b is its size minus its entries with count < min_bucket.
+constexpr int CUT_SMALL_T = 64;
+
+__global__ __launch_bounds__(CUT_NT) void k_count_hist_blocks(const int32_t* TC, uint64_t U, int t,
+                                                             uint32_t* BH) {
+  constexpr int NW = CUT_NT / 64;
+  __shared__ uint32_t h[NW * CUT_SMALL_T];
+  const int w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < NW * t; i += CUT_NT) h[i] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER + (uint64_t)threadIdx.x * 4;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    if (base + i < U) {
+      const int v = TC[base + i];
+      if (v >= 0 && v < t) atomicAdd(&h[w * t + v], 1u);
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < t; c += CUT_NT) {
+    uint32_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < NW; q++) sum += h[q * t + c];
+    BH[(uint64_t)blockIdx.x * t + c] = sum;
+  }
+}
+
+// hist[c] = sum over blocks of BH[b * t + c]; one block per column
+__global__ __launch_bounds__(1024) void k_hist_columns(const uint32_t* BH, uint64_t nb, int t, ull* hist) {
+  const int c = blockIdx.x;
+  ull s = 0;
+  for (uint64_t b = threadIdx.x; b < nb; b += 1024) s += BH[b * t + c];
+  s = block_sum<1024>(s);
+  if (threadIdx.x == 0) hist[c] = s;
+}
+
+// kept[b] = block size - entries with count < min_bucket
+__global__ void k_kept_from_hist(const uint32_t* BH, uint64_t nb, uint64_t U, int t, const int* mb,
+                                 ull* kept) {
+  const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int m = *mb;
+  const uint64_t size = (b + 1) * SCAN_PER <= U ? SCAN_PER : U - b * SCAN_PER;
+  uint64_t below = 0;
+  for (int c = 0; c < m && c < t; c++) below += BH[b * t + c];
+  kept[b] = size - below;
+}
+
 // min_bucket = first argmin over hist[1..t-1]; an empty range gives t.
 __global__ void k_min_bucket(const ull* hist, int t, int* mb) {
   if (threadIdx.x || blockIdx.x) return;
@@ -1236,42 +1322,70 @@ __device__ __forceinline__ void set_key<u128>(MnRec& r, u128 k) {
 }
 
 // List view: 5 sorted lists of node keys, either staged in LDS (lk) or
-// read from KK through pred/succ.
+// read from KK through pred/succ.  List 0 = pred keys (may repeat, up to 4
+// equal), lists 1..4 = succ keys of quarter a = l - 1 (no repeats).
 template <class Key, bool GLOBAL>
 struct MnLists {
   const Key* KK;
   Key msk;
-  const Key* lk;       // LDS copy (GLOBAL == false)
-  ull off[5];          // list start (index into lk, or into KK)
+  const Key* lk;  // LDS copy (GLOBAL == false)
+  ull off[5];     // list start (index into lk, or into KK)
   ull len[5];
-  __device__ __forceinline__ Key at(int l, ull i) const {
+  __device__ __forceinline__ Key at(ull off_l, bool pred, ull i) const {
     if (GLOBAL) {
-      const Key x = KK[off[l] + i];
-      return l == 0 ? mn_pred(x) : (x & msk);
+      const Key x = KK[off_l + i];
+      return pred ? mn_pred(x) : (x & msk);
     }
-    return lk[off[l] + i];
+    return lk[off_l + i];
   }
-  __device__ __forceinline__ ull lb(int l, Key x) const {
-    ull lo = 0, hi = len[l];
-    while (lo < hi) {
-      ull mid = (lo + hi) >> 1;
-      if (at(l, mid) < x) lo = mid + 1; else hi = mid;
-    }
-    return lo;
+};
+
+template <class Key, bool GLOBAL>
+__device__ __forceinline__ ull mn_lb(const MnLists<Key, GLOBAL>& L, ull o, ull n, bool pred, Key x) {
+  ull lo = 0, hi = n;
+  while (lo < hi) {
+    const ull mid = (lo + hi) >> 1;
+    if (L.at(o, pred, mid) < x) lo = mid + 1; else hi = mid;
   }
-  __device__ __forceinline__ ull ub(int l, Key x) const {
-    ull lo = 0, hi = len[l];
-    while (lo < hi) {
-      ull mid = (lo + hi) >> 1;
-      if (at(l, mid) <= x) lo = mid + 1; else hi = mid;
+  return lo;
+}
+template <class Key, bool GLOBAL>
+__device__ __forceinline__ ull mn_ub(const MnLists<Key, GLOBAL>& L, ull o, ull n, bool pred, Key x) {
+  ull lo = 0, hi = n;
+  while (lo < hi) {
+    const ull mid = (lo + hi) >> 1;
+    if (L.at(o, pred, mid) <= x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Positions of x in all five lists, walked forward from the previous
+// element's positions (x never decreases along a thread's chunk of one list).
+template <class Key, bool GLOBAL>
+struct MnCursor {
+  ull lb[5], ub[5];
+  __device__ __forceinline__ void seek(const MnLists<Key, GLOBAL>& L, Key x) {
+#pragma unroll
+    for (int o = 0; o < 5; o++) {
+      lb[o] = mn_lb<Key, GLOBAL>(L, L.off[o], L.len[o], o == 0, x);
+      ub[o] = lb[o];
+      while (ub[o] < L.len[o] && L.at(L.off[o], o == 0, ub[o]) <= x) ub[o]++;
     }
-    return lo;
+  }
+  __device__ __forceinline__ void advance(const MnLists<Key, GLOBAL>& L, Key x) {
+#pragma unroll
+    for (int o = 0; o < 5; o++) {
+      while (lb[o] < L.len[o] && L.at(L.off[o], o == 0, lb[o]) < x) lb[o]++;
+      if (ub[o] < lb[o]) ub[o] = lb[o];
+      while (ub[o] < L.len[o] && L.at(L.off[o], o == 0, ub[o]) <= x) ub[o]++;
+    }
   }
 };
 
 // One block per range.  GLOBAL == false: ranges with e <= MN_ECAP, lists in
 // LDS.  GLOBAL == true: the remaining ranges, lists and rank flags in global
-// memory (gflags, indexed from plan.base).
+// memory (gflags, indexed from plan.base).  Each thread handles a contiguous
+// chunk of the concatenated lists and walks its positions in the other lists.
 template <class Key, bool GLOBAL>
 __global__ __launch_bounds__(MN_NT) void k_mn_build(const Key* KK, const int32_t* KC, int K,
                                                     const MnPlan* plan, uint64_t G, MnRec* NT,
@@ -1289,44 +1403,56 @@ __global__ __launch_bounds__(MN_NT) void k_mn_build(const Key* KK, const int32_t
   L.KK = KK;
   L.msk = key_mask<Key>(2 * (K - 1));
   L.lk = lk;
+  ull cum[6];
   {
     ull o = 0;
+#pragma unroll
     for (int l = 0; l < 5; l++) {
       L.len[l] = p.hi[l] - p.lo[l];
       L.off[l] = GLOBAL ? p.lo[l] : o;
+      cum[l] = o;
       o += L.len[l];
     }
+    cum[5] = o;
   }
   if (!GLOBAL) {
+#pragma unroll
     for (int l = 0; l < 5; l++) {
       for (ull i = threadIdx.x; i < L.len[l]; i += MN_NT) {
         const Key x = KK[p.lo[l] + i];
-        lk[L.off[l] + i] = l == 0 ? mn_pred(x) : (x & L.msk);
+        lk[cum[l] + i] = l == 0 ? mn_pred(x) : (x & L.msk);
       }
     }
     __syncthreads();
   }
   uint32_t* flags = GLOBAL ? gflags + p.base : lflag;
-  // pass 1: merged rank and node-start flag of every element
-  for (ull e = threadIdx.x; e < E; e += MN_NT) {
-    int l = 0;
-    ull j = e;
-    while (j >= L.len[l]) { j -= L.len[l]; l++; }
-    const Key x = L.at(l, j);
-    ull rank = j;
-    bool start = L.lb(l, x) == j;
-    for (int o = 0; o < 5; o++) {
-      if (o == l) continue;
-      const ull lbo = L.lb(o, x);
-      if (o < l) {
-        const ull ubo = L.ub(o, x);
-        rank += ubo;
-        if (ubo != lbo) start = false;
-      } else {
-        rank += lbo;
+  const ull chunk = (E + MN_NT - 1) / MN_NT;
+  const ull e0 = (ull)threadIdx.x * chunk;
+  const ull e1 = e0 + chunk < E ? e0 + chunk : E;
+  // pass 1: merged rank and node-start flag of every element of the chunk
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+    const ull a = e0 > cum[l] ? e0 : cum[l];
+    const ull b = e1 < cum[l + 1] ? e1 : cum[l + 1];
+    if (a >= b) continue;
+    MnCursor<Key, GLOBAL> cur;
+    for (ull e = a; e < b; e++) {
+      const ull j = e - cum[l];
+      const Key x = L.at(L.off[l], l == 0, j);
+      if (e == a) cur.seek(L, x); else cur.advance(L, x);
+      ull rank = j;
+      bool start = cur.lb[l] == j;
+#pragma unroll
+      for (int o = 0; o < 5; o++) {
+        if (o < l) {
+          rank += cur.ub[o];
+          if (cur.ub[o] != cur.lb[o]) start = false;
+        } else if (o > l) {
+          rank += cur.lb[o];
+        }
       }
+      flags[rank] = start ? 1u : 0u;
     }
-    flags[rank] = start ? 1u : 0u;
   }
   __syncthreads();
   // pass 2: exclusive scan of flags in rank order (chunks of MN_NT*4)
@@ -1335,14 +1461,14 @@ __global__ __launch_bounds__(MN_NT) void k_mn_build(const Key* KK, const int32_t
   for (ull c0 = 0; c0 < E; c0 += (ull)MN_NT * 4) {
     const ull i0 = c0 + (ull)threadIdx.x * 4;
     uint32_t v[4];
-    ull s = 0;
+    ull sm = 0;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       v[i] = i0 + i < E ? flags[i0 + i] : 0u;
-      s += v[i];
+      sm += v[i];
     }
-    ull ex = block_excl_scan<MN_NT>(s) + chunk_base;
-    const ull tot = block_sum<MN_NT>(s);
+    ull ex = block_excl_scan<MN_NT>(sm) + chunk_base;
+    const ull tot = block_sum<MN_NT>(sm);
     // store "node index + 1" for starts, 0 otherwise
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -1355,37 +1481,43 @@ __global__ __launch_bounds__(MN_NT) void k_mn_build(const Key* KK, const int32_t
   }
   const ull nodes = chunk_base;
   // pass 3: every node start assembles its record
-  for (ull e = threadIdx.x; e < E; e += MN_NT) {
-    int l = 0;
-    ull j = e;
-    while (j >= L.len[l]) { j -= L.len[l]; l++; }
-    const Key x = L.at(l, j);
-    if (L.lb(l, x) != j) continue;
-    ull rank = j;
-    ull lbs[5], ubs[5];
-    bool start = true;
-    for (int o = 0; o < 5; o++) {
-      lbs[o] = L.lb(o, x);
-      ubs[o] = L.ub(o, x);
-      if (o < l) {
-        rank += ubs[o];
-        if (ubs[o] != lbs[o]) start = false;
-      } else if (o > l) {
-        rank += lbs[o];
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+    const ull a = e0 > cum[l] ? e0 : cum[l];
+    const ull b = e1 < cum[l + 1] ? e1 : cum[l + 1];
+    if (a >= b) continue;
+    MnCursor<Key, GLOBAL> cur;
+    for (ull e = a; e < b; e++) {
+      const ull j = e - cum[l];
+      const Key x = L.at(L.off[l], l == 0, j);
+      if (e == a) cur.seek(L, x); else cur.advance(L, x);
+      if (cur.lb[l] != j) continue;
+      ull rank = j;
+      bool start = true;
+#pragma unroll
+      for (int o = 0; o < 5; o++) {
+        if (o < l) {
+          rank += cur.ub[o];
+          if (cur.ub[o] != cur.lb[o]) start = false;
+        } else if (o > l) {
+          rank += cur.lb[o];
+        }
       }
+      if (!start) continue;
+      const uint32_t idx1 = flags[rank];
+      MnRec rec;
+      set_key<Key>(rec, x);
+#pragma unroll
+      for (int q = 0; q < 4; q++) rec.pre[q] = rec.suf[q] = 0;
+      for (ull i = cur.lb[0]; i < cur.ub[0]; i++) {
+        const ull gi = p.lo[0] + i;
+        rec.suf[(int)(KK[gi] & Key(3))] = KC[gi];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (cur.ub[q + 1] > cur.lb[q + 1]) rec.pre[q] = KC[p.lo[q + 1] + cur.lb[q + 1]];
+      NT[p.base + idx1 - 1] = rec;
     }
-    if (!start) continue;
-    const uint32_t idx1 = flags[rank];
-    MnRec rec;
-    set_key<Key>(rec, x);
-    for (int b = 0; b < 4; b++) rec.pre[b] = rec.suf[b] = 0;
-    for (ull i = lbs[0]; i < ubs[0]; i++) {
-      const ull gi = p.lo[0] + i;
-      rec.suf[(int)(KK[gi] & Key(3))] = KC[gi];
-    }
-    for (int a = 0; a < 4; a++)
-      if (ubs[a + 1] > lbs[a + 1]) rec.pre[a] = KC[p.lo[a + 1] + lbs[a + 1]];
-    NT[p.base + idx1 - 1] = rec;
   }
   if (threadIdx.x == 0) ng[g] = nodes;
 }
@@ -1412,7 +1544,7 @@ enum BufId {
   B_FASTA, B_BLK, B_BLKOFF, B_NL, B_RSTART, B_RLEN, B_KC, B_KOFS, B_SMALL, B_BCNT, B_BOFF, B_BCUR,
   B_S, B_C, B_TU, B_UOFF, B_TK, B_TC, B_HIST, B_KBS, B_KBOFF, B_KK, B_KC2, B_MNQ, B_MNPLAN,
   B_MNNT, B_MNNG, B_MNNOFF, B_MNFLAG, B_NODES, B_TMP, B_OVF, B_PT0, B_PT1, B_PT2, B_PT3, B_PT4,
-  B_PT5, B_PT6, B_BLKGT, B_S2, B_BCNT2, B_BOFF2, B_BCUR2, B_TP, B_HIST1, B_OFF1, B_HIST2, B_OFF2, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
+  B_PT5, B_PT6, B_BLKGT, B_S2, B_BCNT2, B_BOFF2, B_BCUR2, B_TP, B_HIST1, B_OFF1, B_HIST2, B_OFF2, B_CUTBH, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
 };
 
 }  // namespace
@@ -1925,19 +2057,37 @@ int cutoff_impl(pk_ctx* c, int t, uint64_t* hist_out, int* mb_out, uint64_t* nk_
   PK_TRY(need(c, B_SMALL, 16, &small));
   PK_TRY(fill_u64(c, hist, 0, (uint64_t)t));
   const uint64_t nb = (U + SCAN_PER - 1) / SCAN_PER;
-  if (nb) {
-    hipLaunchKernelGGL(k_count_hist, dim3((unsigned)nb), dim3(CUT_NT), 0, st, TC, U, t, hist);
-    PK_HIP(c, hipGetLastError());
-  }
   int* mbd = reinterpret_cast<int*>(small + 8);
-  hipLaunchKernelGGL(k_min_bucket, dim3(1), dim3(1), 0, st, hist, t, mbd);
-  PK_HIP(c, hipGetLastError());
   ull *bs, *boffs;
   PK_TRY(need(c, B_KBS, nb + 1, &bs));
   PK_TRY(need(c, B_KBOFF, nb + 1, &boffs));
-  if (nb) {
-    hipLaunchKernelGGL(k_kept_count, dim3((unsigned)nb), dim3(CUT_NT), 0, st, TC, U, mbd, bs);
+  if (t <= CUT_SMALL_T) {
+    uint32_t* BH;
+    PK_TRY(need(c, B_CUTBH, nb * (uint64_t)t + 1, &BH));
+    if (nb) {
+      hipLaunchKernelGGL(k_count_hist_blocks, dim3((unsigned)nb), dim3(CUT_NT), 0, st, TC, U, t, BH);
+      PK_HIP(c, hipGetLastError());
+      hipLaunchKernelGGL(k_hist_columns, dim3((unsigned)t), dim3(1024), 0, st, BH, nb, t, hist);
+      PK_HIP(c, hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_min_bucket, dim3(1), dim3(1), 0, st, hist, t, mbd);
     PK_HIP(c, hipGetLastError());
+    if (nb) {
+      hipLaunchKernelGGL(k_kept_from_hist, dim3(grid_for(nb, 256)), dim3(256), 0, st, BH, nb, U, t,
+                         mbd, bs);
+      PK_HIP(c, hipGetLastError());
+    }
+  } else {
+    if (nb) {
+      hipLaunchKernelGGL(k_count_hist, dim3((unsigned)nb), dim3(CUT_NT), 0, st, TC, U, t, hist);
+      PK_HIP(c, hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_min_bucket, dim3(1), dim3(1), 0, st, hist, t, mbd);
+    PK_HIP(c, hipGetLastError());
+    if (nb) {
+      hipLaunchKernelGGL(k_kept_count, dim3((unsigned)nb), dim3(CUT_NT), 0, st, TC, U, mbd, bs);
+      PK_HIP(c, hipGetLastError());
+    }
   }
   PK_TRY(scan_excl<ull>(c, bs, boffs, nb));
   ull V = 0;
