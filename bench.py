@@ -33,11 +33,16 @@ WORKLOADS = {
 }
 
 
+# stages timed around exactly one kernel launch -> that kernel's name
+KERNELS = {"hist": "k_l1_hist<{key}>", "l1_scatter": "k_l1_scatter<{key}>",
+           "sort": "k_bucket_hash<{key}>", "compact": "k_compact<{key}>"}
+
+
 def alg_bytes(N, U, V, M, S, W):
     """Algorithmic HBM bytes per kernel (each datum read or written once)."""
     return {
         "hist": S,                              # read the FASTA
-        "scatter": S + N * W,                   # read the FASTA, write the instances
+        "l1_scatter": S + N * W,                # read the FASTA, write the instances
         "sort": N * W + U * (W + 4),            # read the instances, write (key, count)
         "compact": 2 * U * (W + 4),             # read and write the unique table
         "cutoff": U * 4 + U * (W + 4) + V * (W + 4),
@@ -73,15 +78,16 @@ def cpu_baseline(fa, k, t, frac):
                       f"{dt:.1f} s"}
 
 
-def load_traffic(kernel, workload):
+def load_traffic(stage, workload):
+    """HBM bytes per launch of the stage's kernel, from the rocprofv3 PMC runs
+    (FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a plain copy of
+    known size) summarised in profiles/traffic.json; None when not measured."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
         return None
-    try:
-        d = json.load(open(p))
-        return d.get(f"{kernel}:{workload}")
-    except Exception:
-        return None
+    with open(p) as f:
+        e = json.load(f).get(f"{stage}:{workload}")
+    return float(e["bytes"]) if e else None
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):
@@ -138,7 +144,8 @@ def main():
 
     for _ in range(args.warmup):
         step()
-    keys = ("parse_ms", "hist_ms", "scatter_ms", "sort_ms", "split_ms", "compact_ms", "cutoff_ms", "mn_ms")
+    keys = ("parse_ms", "hist_ms", "scatter_ms", "l1_scatter_ms", "sort_ms", "split_ms", "compact_ms",
+            "cutoff_ms", "mn_ms")
     acc = {x: 0.0 for x in keys}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -158,8 +165,9 @@ def main():
     value = N * args.steps / elapsed
     avg = {x.replace("_ms", ""): acc[x] / args.steps for x in keys}
     ab = alg_bytes(N, U, V, M, nbytes, W)
-    # dominant kernel: the longest per-launch time among the single-kernel stages
-    cand = {x: avg[x] for x in ("hist", "scatter", "sort", "compact") if avg[x] > 0}
+    # dominant kernel: the longest per-launch time among the stages timed
+    # around exactly one kernel launch
+    cand = {x: avg[x] for x in KERNELS if avg[x] > 0}
     dom = max(cand, key=cand.get)
     achieved = ab[dom] / (avg[dom] * 1e-3)
     traffic = load_traffic(dom, args.workload)
@@ -180,7 +188,8 @@ def main():
                    "kmer_instances": N, "unique_kmers": U, "kept_kmers": V, "macro_nodes": M,
                    "fasta_bytes": nbytes, "bucket_bits": int(tm["bucket_bits"]),
                    "split_buckets": int(tm["n_split"]), "parallelism": "1 GPU"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+        "roofline": {"bound": "hbm", "kernel": KERNELS[dom].format(key="unsigned long" if k <= 32 else "u128"),
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "alg_bytes_per_launch": ab[dom], "avg_ms": avg[dom]},
         "pipeline": {"alg_bytes": ab["pipeline"], "frac_of_peak": ab["pipeline"] / (ms_per_step * 1e-3) / HBM_PEAK,

@@ -17,8 +17,6 @@
  *   pk_builder_count    Concurrent_Kmer_Partial_Counter<kmer_t>::
  *                       default_count_and_process(processor)
  *                       src/concurrent_kmer_counter.hpp:2018-2082
- *   pk_batch_count      no single counterpart: many independent read sets,
- *                       each with the pk_count semantics, in one call
  *
  * Threading: a context is used by one host thread at a time; contexts are
  * independent (each owns a HIP stream and its device memory).
@@ -91,6 +89,7 @@ typedef struct pk_timing {
   uint64_t n_reads;     /* reads with at least one byte of sequence */
   int32_t bucket_bits;  /* partition width used by the last pk_count */
   int32_t n_split;      /* buckets that needed splitting */
+  float l1_scatter_ms;  /* first scatter level alone (one kernel, part of scatter_ms) */
 } pk_timing;
 
 /* Create a context on HIP device `device`.  hbm_budget_bytes = 0 means no
@@ -109,6 +108,25 @@ uint64_t pk_ctx_device_bytes(const pk_ctx* ctx);
  * resident.  8 <= k <= 64.  On success *out describes the table. */
 int pk_count(pk_ctx* ctx, const char* fasta, uint64_t nbytes, int k, int fasta_on_device,
              pk_table* out);
+
+/* Builder-API counting (Concurrent_Kmer_Partial_Counter<kmer_t>::
+ * default_count_and_process, src/concurrent_kmer_counter.hpp:2018-2082, with
+ * the ctor arguments klen, read_len, delimiter ('\n') and skip_symbol ('>')
+ * of :49-56): from each skip symbol that is the first after a delimiter (or
+ * in the buffer), skip to the next delimiter and take read_len bytes (fewer
+ * at the end of the buffer) as the read; reads shorter than klen are skipped.
+ * delimiter != skip_symbol.
+ * Bases A C T G -> 0 1 2 3, any other byte -> 0 (symbol_to_kmer, :2097-2110);
+ * the key is the whole kmer_t word (kmer_bits = 64 for uint64_t/size_t, 128
+ * for unsigned __int128) after shifting in every byte of the window, so it
+ * keeps the last kmer_bits/2 bases.  The table holds every distinct key with
+ * its count, ascending (the reference hands the same pairs to its processor
+ * in hash order); *n_reads = the reference's return value (reads counted).
+ * out->k = klen.  pk_cutoff applies to the table; pk_macro_nodes only when
+ * klen == kmer_bits / 2. */
+int pk_builder_count(pk_ctx* ctx, const char* fasta, uint64_t nbytes, int klen, uint64_t read_len,
+                     int kmer_bits, char delimiter, char skip_symbol, int fasta_on_device,
+                     pk_table* out, uint64_t* n_reads);
 
 /* Histogram cutoff on the current table (kmer_histogram.cpp:84-140):
  * hist_t[c] (host array of t entries) = number of k-mers with count c < t;

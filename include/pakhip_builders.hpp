@@ -1,0 +1,205 @@
+// pakhip_builders.hpp -- C++ adapters with the shapes of the reference's
+// builder interfaces, over the C ABI of pakhip.h (libpakhip.so).
+//
+//   pakhip::Hip_Kmer_Partial_Counter<kmer_t>
+//       drop-in for Concurrent_Kmer_Partial_Counter<kmer_t>
+//       (src/concurrent_kmer_counter.hpp:21-68 ctor / reset, :2018-2082
+//       default_count_and_process).  Same constructor arguments, same
+//       return value (reads counted), same processor protocol
+//       (processor(begin, end) over pairs with ->first = k-mer word,
+//       ->second = int count), std::logic_error on failure.
+//   pakhip::count_kmers / pakhip::kmer_histogram / pakhip::macro_nodes
+//       the main path (perform_kmer_counting, src/kmer_counting.cpp:3351;
+//       construct_kmer_histogram, src/kmer_histogram.cpp:72;
+//       construct_macro_nodes, src/mn_construct.cpp:212) as value-returning
+//       calls on one device context.
+//
+// Header-only; link with -lpakhip.  Host code only (no HIP headers needed).
+#ifndef PAKHIP_BUILDERS_HPP
+#define PAKHIP_BUILDERS_HPP
+
+#include <cstddef>
+#include <cstdint>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "pakhip.h"
+
+namespace pakhip {
+
+inline void throw_on_error(pk_ctx* ctx, int rc, const char* what) {
+  if (rc == PK_OK) return;
+  std::string msg = std::string(what) + " failed (" + std::to_string(rc) + ")";
+  if (ctx) msg += ": " + std::string(pk_last_error(ctx));
+  throw std::logic_error(msg);
+}
+
+// One device context (HIP stream + device memory); not copyable.
+class Device_Context {
+ public:
+  explicit Device_Context(int device = 0, uint64_t hbm_budget_bytes = 0) {
+    throw_on_error(nullptr, pk_ctx_create(&ctx_, device, hbm_budget_bytes), "pk_ctx_create");
+  }
+  ~Device_Context() { pk_ctx_destroy(ctx_); }
+  Device_Context(const Device_Context&) = delete;
+  Device_Context& operator=(const Device_Context&) = delete;
+  pk_ctx* get() const { return ctx_; }
+
+ private:
+  pk_ctx* ctx_ = nullptr;
+};
+
+// kmer_t -> key words of the table (1: 64-bit word, 2: 128-bit word).
+template <class kmer_t>
+struct kmer_word_traits {
+  static_assert(sizeof(kmer_t) == 8 || sizeof(kmer_t) == 16, "kmer_t must be a 64- or 128-bit word");
+  static constexpr int bits = (int)sizeof(kmer_t) * 8;
+  static kmer_t from_words(const uint64_t* w) {
+    if (sizeof(kmer_t) == 8) return (kmer_t)w[0];
+    return (kmer_t)(((unsigned __int128)w[1] << 64) | (unsigned __int128)w[0]);
+  }
+};
+
+template <class kmer_t, template <class> class alloc = std::allocator>
+class Hip_Kmer_Partial_Counter {
+ public:
+  typedef std::pair<kmer_t, int> kmer_pair_t;
+
+  Hip_Kmer_Partial_Counter(size_t klen = 32, const char* symbol_begin = NULL,
+                           const char* symbol_end = NULL, size_t read_len = 100,
+                           size_t max_kmer_count = 1000000, char delimiter = '\n',
+                           char skip_symbol = '>', int device = 0)
+      : klen_(klen), symbol_begin_(symbol_begin), symbol_end_(symbol_end), read_len_(read_len),
+        max_kmer_count_(max_kmer_count), delimiter_(delimiter), skip_symbol_(skip_symbol),
+        ctx_(device) {}
+
+  void reset(char const* begin, char const* end) {
+    symbol_begin_ = begin;
+    symbol_end_ = end;
+  }
+
+  // Counts on the device, then hands the ascending (k-mer, count) pairs to
+  // the processor: one call when they number at most max_kmer_count (the
+  // reference's trailing call; every reference caller passes SIZE_MAX),
+  // otherwise consecutive slices of max_kmer_count pairs.  Returns the
+  // number of reads counted.
+  template <typename processor_t>
+  size_t default_count_and_process(processor_t& processor) {
+    const size_t n = (symbol_begin_ && symbol_end_ && symbol_end_ > symbol_begin_)
+                         ? (size_t)(symbol_end_ - symbol_begin_)
+                         : 0;
+    if (klen_ > (size_t)std::numeric_limits<int>::max())
+      throw std::logic_error("klen too large");
+    pk_table t;
+    uint64_t reads = 0;
+    throw_on_error(ctx_.get(),
+                   pk_builder_count(ctx_.get(), n ? symbol_begin_ : nullptr, n, (int)klen_,
+                                    read_len_, kmer_word_traits<kmer_t>::bits, delimiter_,
+                                    skip_symbol_, 0, &t, &reads),
+                   "pk_builder_count");
+    std::vector<uint64_t> words((size_t)t.n * t.words);
+    std::vector<int32_t> counts((size_t)t.n);
+    if (t.n)
+      throw_on_error(ctx_.get(), pk_download_table(ctx_.get(), words.data(), counts.data()),
+                     "pk_download_table");
+    std::vector<kmer_pair_t, alloc<kmer_pair_t>> pairs;
+    pairs.reserve((size_t)t.n);
+    for (size_t i = 0; i < (size_t)t.n; i++)
+      pairs.emplace_back(kmer_word_traits<kmer_t>::from_words(&words[i * t.words]), counts[i]);
+    const size_t step = max_kmer_count_ ? max_kmer_count_ : 1;
+    if (pairs.size() <= step) {
+      processor(pairs.cbegin(), pairs.cend());
+    } else {
+      for (size_t b = 0; b < pairs.size(); b += step) {
+        const size_t e = b + step < pairs.size() ? b + step : pairs.size();
+        processor(pairs.cbegin() + b, pairs.cbegin() + e);
+      }
+    }
+    symbol_begin_ = symbol_end_;  // the reference's scan consumes the range
+    return (size_t)reads;
+  }
+
+  // The reference's sibling counters produce the same table; on the device
+  // they are one implementation.
+  template <typename processor_t>
+  size_t default_count_and_process_sort(processor_t& processor) {
+    return default_count_and_process(processor);
+  }
+
+  pk_ctx* context() const { return ctx_.get(); }
+
+ private:
+  size_t klen_;
+  const char* symbol_begin_;
+  const char* symbol_end_;
+  size_t read_len_;
+  size_t max_kmer_count_;
+  char delimiter_;
+  char skip_symbol_;
+  Device_Context ctx_;
+};
+
+// -- main path -------------------------------------------------------------
+
+struct Kmer_Table {
+  int k = 0;
+  int words = 1;
+  std::vector<uint64_t> keys;  // n * words, lo word first
+  std::vector<int32_t> counts;
+  size_t size() const { return counts.size(); }
+};
+
+// perform_kmer_counting (src/kmer_counting.cpp:3351) on one device: the
+// sorted unique (k-mer, count) table the reference leaves in kmer_proc_buf.
+inline Kmer_Table count_kmers(pk_ctx* ctx, const char* read_data, size_t length, int k) {
+  pk_table t;
+  throw_on_error(ctx, pk_count(ctx, read_data, length, k, 0, &t), "pk_count");
+  Kmer_Table out;
+  out.k = t.k;
+  out.words = t.words;
+  out.keys.resize((size_t)t.n * t.words);
+  out.counts.resize((size_t)t.n);
+  if (t.n) throw_on_error(ctx, pk_download_table(ctx, out.keys.data(), out.counts.data()), "pk_download_table");
+  return out;
+}
+
+struct Histogram_Cutoff {
+  std::vector<uint64_t> hist;  // hist[c] for c < t
+  int min_bucket = 0;
+  Kmer_Table kept;
+};
+
+// construct_kmer_histogram (src/kmer_histogram.cpp:72-249, one rank) on the
+// table of the last count_kmers.  num_buckets == 0 means 21.
+inline Histogram_Cutoff kmer_histogram(pk_ctx* ctx, int num_buckets, int k) {
+  Histogram_Cutoff r;
+  const int t = num_buckets == 0 ? 21 : num_buckets;
+  r.hist.assign((size_t)(t > 0 ? t : 1), 0);
+  uint64_t nk = 0;
+  throw_on_error(ctx, pk_cutoff(ctx, num_buckets, r.hist.data(), &r.min_bucket, &nk), "pk_cutoff");
+  pk_table kt;
+  throw_on_error(ctx, pk_kept_table(ctx, &kt), "pk_kept_table");
+  r.kept.k = k;
+  r.kept.words = kt.words;
+  r.kept.keys.resize((size_t)kt.n * kt.words);
+  r.kept.counts.resize((size_t)kt.n);
+  if (kt.n) throw_on_error(ctx, pk_download_kept(ctx, r.kept.keys.data(), r.kept.counts.data()), "pk_download_kept");
+  return r;
+}
+
+// construct_macro_nodes (src/mn_construct.cpp:212-477, one rank) on the kept
+// table of the last kmer_histogram: node records ascending by (k-1)-mer key.
+inline std::vector<pk_mnode> macro_nodes(pk_ctx* ctx) {
+  pk_mn_table m;
+  throw_on_error(ctx, pk_macro_nodes(ctx, &m), "pk_macro_nodes");
+  std::vector<pk_mnode> nodes((size_t)m.n);
+  if (m.n) throw_on_error(ctx, pk_download_nodes(ctx, nodes.data()), "pk_download_nodes");
+  return nodes;
+}
+
+}  // namespace pakhip
+
+#endif  // PAKHIP_BUILDERS_HPP

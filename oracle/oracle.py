@@ -46,7 +46,7 @@ def lib():
         pp = ctypes.POINTER(ctypes.c_void_p)
         L.pko_count.argtypes = [vp, sz, i32, i32, i32, u64, i32, pp]
         L.pko_bruteforce_count.argtypes = [vp, sz, i32, pp]
-        L.pko_builder_count.argtypes = [vp, sz, i32, u64, i32, pp]
+        L.pko_builder_count.argtypes = [vp, sz, i32, u64, i32, ctypes.c_char, ctypes.c_char, pp]
         for f in ("pko_count", "pko_bruteforce_count", "pko_builder_count", "pko_cutoff",
                   "pko_macro_nodes", "pko_lmer_frequency", "pko_lmer_owner", "pko_visit_value"):
             getattr(L, f).restype = i32
@@ -121,11 +121,13 @@ def bruteforce_count(fasta, k: int):
     return keys, counts
 
 
-def builder_count(fasta, klen: int, read_len: int = 100, kmer_bits: int = 64):
+def builder_count(fasta, klen: int, read_len: int = 100, kmer_bits: int = 64,
+                  delimiter: bytes = b"\n", skip_symbol: bytes = b">"):
     """(reads, keys, counts) of Concurrent_Kmer_Partial_Counter::default_count_and_process."""
     a, n = _buf(fasta)
     h = ctypes.c_void_p()
-    rc = lib().pko_builder_count(a.ctypes.data, n, klen, read_len, kmer_bits, ctypes.byref(h))
+    rc = lib().pko_builder_count(a.ctypes.data, n, klen, read_len, kmer_bits, delimiter, skip_symbol,
+                                 ctypes.byref(h))
     if rc:
         raise OracleError(rc, "pko_builder_count")
     keys, counts, reads = _take_table(h)

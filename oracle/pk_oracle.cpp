@@ -333,8 +333,8 @@ void bruteforce(const char* s, size_t n, int k, std::map<K, long long>& out) {
 // calls, which is what an aggregating processor sees (parity of the partial
 // call boundaries is unpinned and not reproduced).
 template <class K>
-size_t builder_count(const char* beg, const char* end, size_t klen, size_t read_len,
-                     std::map<K, long long>& out) {
+size_t builder_count(const char* beg, const char* end, size_t klen, size_t read_len, char delim,
+                     char skip, std::map<K, long long>& out) {
   auto sym = [](char c) -> K {
     switch (c) {
       case 'A': return 0;
@@ -347,10 +347,10 @@ size_t builder_count(const char* beg, const char* end, size_t klen, size_t read_
   size_t rid = 0;
   const char* cur = beg;
   while (cur != end) {
-    while (cur != end && *cur != '>') ++cur;
+    while (cur != end && *cur != skip) ++cur;
     if (cur == end) break;
     ++cur;
-    while (cur != end && *cur != '\n') ++cur;
+    while (cur != end && *cur != delim) ++cur;
     if (cur == end) break;
     ++cur;
     size_t rl = read_len;
@@ -555,17 +555,17 @@ int pko_bruteforce_count(const char* fasta, size_t len, int k, pko_table** out) 
 // Builder-API table (default_count_and_process).  kmer_bits = 64 or 128 is
 // the width of the counter's kmer_t template argument.
 int pko_builder_count(const char* fasta, size_t len, int klen, uint64_t read_len, int kmer_bits,
-                      pko_table** out) {
+                      char delim, char skip, pko_table** out) {
   *out = nullptr;
   if (klen < 1 || (kmer_bits != 64 && kmer_bits != 128)) return E_INVAL;
   Table* t = new Table();
   if (kmer_bits == 64) {
     std::map<uint64_t, long long> m;
-    t->reads = builder_count<uint64_t>(fasta, fasta + len, (size_t)klen, read_len, m);
+    t->reads = builder_count<uint64_t>(fasta, fasta + len, (size_t)klen, read_len, delim, skip, m);
     for (auto& kv : m) { put_key(t->keys, kv.first); t->counts.push_back((int32_t)kv.second); }
   } else {
     std::map<u128, long long> m;
-    t->reads = builder_count<u128>(fasta, fasta + len, (size_t)klen, read_len, m);
+    t->reads = builder_count<u128>(fasta, fasta + len, (size_t)klen, read_len, delim, skip, m);
     t->words = 2;
     for (auto& kv : m) { put_key(t->keys, kv.first); t->counts.push_back((int32_t)kv.second); }
   }

@@ -46,7 +46,8 @@ class _Timing(ctypes.Structure):
                 ("split_ms", ctypes.c_float), ("compact_ms", ctypes.c_float),
                 ("cutoff_ms", ctypes.c_float), ("mn_ms", ctypes.c_float),
                 ("n_instances", ctypes.c_uint64), ("n_reads", ctypes.c_uint64),
-                ("bucket_bits", ctypes.c_int32), ("n_split", ctypes.c_int32)]
+                ("bucket_bits", ctypes.c_int32), ("n_split", ctypes.c_int32),
+                ("l1_scatter_ms", ctypes.c_float)]
 
 
 MNODE_DTYPE = np.dtype([("key_lo", "<u8"), ("key_hi", "<u8"), ("pre_cnt", "<i4", (4,)),
@@ -54,8 +55,9 @@ MNODE_DTYPE = np.dtype([("key_lo", "<u8"), ("key_hi", "<u8"), ("pre_cnt", "<i4",
 
 # every symbol declared in include/pakhip.h
 EXPORTS = ("pk_ctx_create", "pk_ctx_destroy", "pk_last_error", "pk_ctx_stream",
-           "pk_ctx_device_bytes", "pk_count", "pk_cutoff", "pk_kept_table", "pk_macro_nodes",
-           "pk_download_table", "pk_download_kept", "pk_download_nodes", "pk_last_timing")
+           "pk_ctx_device_bytes", "pk_count", "pk_builder_count", "pk_cutoff", "pk_kept_table",
+           "pk_macro_nodes", "pk_download_table", "pk_download_kept", "pk_download_nodes",
+           "pk_last_timing")
 
 _LIB = None
 
@@ -88,6 +90,9 @@ def load_library():
     L.pk_ctx_device_bytes.restype = u64
     L.pk_count.argtypes = [vp, vp, u64, i32, i32, ctypes.POINTER(_Table)]
     L.pk_count.restype = i32
+    L.pk_builder_count.argtypes = [vp, vp, u64, i32, u64, i32, ctypes.c_char, ctypes.c_char, i32,
+                                   ctypes.POINTER(_Table), ctypes.POINTER(u64)]
+    L.pk_builder_count.restype = i32
     L.pk_cutoff.argtypes = [vp, i32, vp, ctypes.POINTER(i32), ctypes.POINTER(u64)]
     L.pk_cutoff.restype = i32
     L.pk_kept_table.argtypes = [vp, ctypes.POINTER(_Table)]
@@ -104,6 +109,12 @@ def load_library():
     L.pk_last_timing.restype = i32
     _LIB = L
     return L
+
+
+def _as_u8(fasta) -> np.ndarray:
+    if isinstance(fasta, (bytes, bytearray)):
+        return np.frombuffer(fasta, dtype=np.uint8)
+    return np.ascontiguousarray(fasta, dtype=np.uint8)
 
 
 @dataclass
@@ -168,12 +179,34 @@ class Context:
         if device_ptr is not None:
             rc = self._lib.pk_count(self._h, ctypes.c_void_p(device_ptr), nbytes, k, 1, ctypes.byref(t))
         else:
-            a = np.frombuffer(fasta, dtype=np.uint8) if isinstance(fasta, (bytes, bytearray)) else np.ascontiguousarray(fasta, dtype=np.uint8)
+            a = _as_u8(fasta)
             rc = self._lib.pk_count(self._h, a.ctypes.data, a.nbytes, k, 0, ctypes.byref(t))
         self._check(rc)
         self.k = k
         self.table = TableView(int(t.n), int(t.words), int(t.k), t.keys or 0, t.counts or 0)
         return self.table
+
+    # -- Concurrent_Kmer_Partial_Counter::default_count_and_process ------------
+    def builder_count(self, fasta, klen: int, read_len: int = 100, kmer_bits: int = 64,
+                      delimiter: bytes = b"\n", skip_symbol: bytes = b">",
+                      device_ptr: int | None = None, nbytes: int | None = None) -> int:
+        """Builder-API count (concurrent_kmer_counter.hpp:2018-2082); returns the
+        number of reads counted, the table is self.table."""
+        t = _Table()
+        nr = ctypes.c_uint64()
+        if device_ptr is not None:
+            rc = self._lib.pk_builder_count(self._h, ctypes.c_void_p(device_ptr), nbytes, klen, read_len,
+                                            kmer_bits, delimiter, skip_symbol, 1, ctypes.byref(t),
+                                            ctypes.byref(nr))
+        else:
+            a = _as_u8(fasta)
+            rc = self._lib.pk_builder_count(self._h, a.ctypes.data, a.nbytes, klen, read_len, kmer_bits,
+                                            delimiter, skip_symbol, 0, ctypes.byref(t),
+                                            ctypes.byref(nr))
+        self._check(rc)
+        self.k = klen
+        self.table = TableView(int(t.n), int(t.words), int(t.k), t.keys or 0, t.counts or 0)
+        return int(nr.value)
 
     def table_host(self):
         """(keys[n, words] uint64, counts[n] int32) of the current table."""
@@ -199,7 +232,7 @@ class Context:
         return hist, int(mb.value), int(nk.value)
 
     def kept_host(self):
-        w = 1 if self.k <= 32 else 2
+        w = self.table.words
         keys = np.empty((self.n_kept, w), dtype=np.uint64)
         counts = np.empty(self.n_kept, dtype=np.int32)
         self._check(self._lib.pk_download_kept(self._h, keys.ctypes.data, counts.ctypes.data))

@@ -2,6 +2,7 @@
 
   _lib/libpakhip.so  HIP kernels + C ABI (include/pakhip.h), gfx950 only
   _lib/libpkgen.so   host generator of synthetic read sets
+  tests/cpp/_bin/builder_adapter_test  host driver of include/pakhip_builders.hpp
 
 Run `python -m pakman_plus_amd.build` or call build_all().
 """
@@ -56,12 +57,27 @@ def build_hip() -> str:
     return out
 
 
+def build_adapter_test() -> str:
+    """Host-only driver of the C++ builder adapter (include/pakhip_builders.hpp),
+    linked against _lib/libpakhip.so; used by the GPU tests."""
+    src = os.path.join(ROOT, "tests", "cpp", "builder_adapter_test.cpp")
+    outdir = os.path.join(ROOT, "tests", "cpp", "_bin")
+    os.makedirs(outdir, exist_ok=True)
+    out = os.path.join(outdir, "builder_adapter_test")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", src, f"-L{LIBDIR}", "-lpakhip",
+          "-Wl,-rpath,$ORIGIN/../../../pakman_plus_amd/_lib", "-Wl,-rpath,/opt/rocm/lib", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all() -> None:
     build_gen()
     build_hip()
+    build_adapter_test()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "gen": build_gen, "hip": build_hip}[w]()
+        {"all": build_all, "gen": build_gen, "hip": build_hip, "adapter": build_adapter_test}[w]()

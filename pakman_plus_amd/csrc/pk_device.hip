@@ -233,8 +233,10 @@ __device__ __forceinline__ void load16(const uint8_t* fa, uint64_t n, uint64_t b
   }
 }
 
-__global__ __launch_bounds__(NL_NT) void k_nl_count(const uint8_t* fa, uint64_t n, ull* blk_cnt,
-                                                    ull* blk_gt) {
+// Event bytes of the index: bytes equal to e0 or e1 ('\n' and '\n' for the
+// line index of the main path; delimiter and skip symbol for the builder).
+__global__ __launch_bounds__(NL_NT) void k_nl_count(const uint8_t* fa, uint64_t n, uint8_t e0,
+                                                    uint8_t e1, ull* blk_cnt, ull* blk_gt) {
   const uint64_t base = (uint64_t)blockIdx.x * NL_PER_BLOCK + (uint64_t)threadIdx.x * NL_PER_THREAD;
   uint8_t b[16];
   load16(fa, n, base, b);
@@ -242,7 +244,7 @@ __global__ __launch_bounds__(NL_NT) void k_nl_count(const uint8_t* fa, uint64_t 
 #pragma unroll
   for (int i = 0; i < 16; i++) {
     if (base + i < n) {
-      c += (b[i] == '\n');
+      c += (b[i] == e0 || b[i] == e1);
       if (b[i] == '>' && gt == ~0ull) gt = base + i;
     }
   }
@@ -262,18 +264,66 @@ __global__ __launch_bounds__(1024) void k_min_reduce(const ull* in, uint64_t n, 
   if (threadIdx.x == 0) *out = m;
 }
 
-__global__ __launch_bounds__(NL_NT) void k_nl_write(const uint8_t* fa, uint64_t n, const ull* blk_off,
-                                                    ull* nl) {
+__global__ __launch_bounds__(NL_NT) void k_nl_write(const uint8_t* fa, uint64_t n, uint8_t e0,
+                                                    uint8_t e1, const ull* blk_off, ull* nl) {
   const uint64_t base = (uint64_t)blockIdx.x * NL_PER_BLOCK + (uint64_t)threadIdx.x * NL_PER_THREAD;
   uint8_t b[16];
   load16(fa, n, base, b);
   ull c = 0;
 #pragma unroll
-  for (int i = 0; i < 16; i++) c += (base + i < n && b[i] == '\n');
+  for (int i = 0; i < 16; i++) c += (base + i < n && (b[i] == e0 || b[i] == e1));
   ull o = block_excl_scan<NL_NT>(c) + blk_off[blockIdx.x];
 #pragma unroll
   for (int i = 0; i < 16; i++)
-    if (base + i < n && b[i] == '\n') nl[o++] = base + i;
+    if (base + i < n && (b[i] == e0 || b[i] == e1)) nl[o++] = base + i;
+}
+
+// Builder-API records (Concurrent_Kmer_Partial_Counter::
+// default_count_and_process, concurrent_kmer_counter.hpp:2018-2082): from
+// the current position the scan finds the next skip symbol ('>'), skips to
+// the delimiter ('\n') that ends the header, and takes read_len bytes from
+// there (fewer at the end of the buffer) as the read; the next scan starts
+// at the read's first byte.  The records are therefore the first skip symbol
+// after every delimiter (and the first one of the buffer): with ev = the
+// positions of all delimiter and skip bytes in order, event e is a record
+// when it is a skip symbol and event e-1 is not.  A record whose header has
+// no delimiter ends the scan; a record is counted when its read holds >=
+// klen bytes.  The two symbols differ (checked on the host).
+struct BldSyms {
+  uint8_t delim, skip;
+};
+
+__device__ __forceinline__ bool bld_record(const uint8_t* fa, uint64_t n, const ull* ev, uint64_t m,
+                                           uint64_t e, int klen, uint64_t read_len, BldSyms sy,
+                                           uint64_t* rb, uint64_t* rl) {
+  if (fa[ev[e]] != sy.skip || (e > 0 && fa[ev[e - 1]] == sy.skip)) return false;
+  uint64_t e2 = e + 1;
+  while (e2 < m && fa[ev[e2]] != sy.delim) e2++;
+  if (e2 >= m) return false;
+  *rb = ev[e2] + 1;
+  *rl = read_len < n - *rb ? read_len : n - *rb;
+  return *rl >= (uint64_t)klen;
+}
+
+__global__ void k_bld_flags(const uint8_t* fa, uint64_t n, const ull* ev, uint64_t m, int klen,
+                            uint64_t read_len, BldSyms sy, uint32_t* flag) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  uint64_t rb, rl;
+  flag[e] = bld_record(fa, n, ev, m, e, klen, read_len, sy, &rb, &rl) ? 1u : 0u;
+}
+
+__global__ void k_bld_write(const uint8_t* fa, uint64_t n, const ull* ev, uint64_t m, int klen,
+                            uint64_t read_len, BldSyms sy, const ull* idx, ull* rstart,
+                            uint32_t* rlen, uint32_t* kc) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  uint64_t rb, rl;
+  if (!bld_record(fa, n, ev, m, e, klen, read_len, sy, &rb, &rl)) return;
+  const ull r = idx[e];
+  rstart[r] = rb;
+  rlen[r] = (uint32_t)rl;
+  kc[r] = (uint32_t)(rl - (uint64_t)klen + 1);
 }
 
 // Records (kmer_counting.cpp:2920-2945).  Candidate r uses header newline
@@ -337,9 +387,26 @@ __device__ __forceinline__ uint64_t load_word8(const uint8_t* fa, uint64_t n, ui
   return w;
 }
 
+// Base encodings of the two counting front ends.
+//   ENC_MAIN:    char_to_el, (c >> 1) & 7 (distribute_kmers.h:784-786): the
+//                3-bit code ORs into the previous base's low bit, and the
+//                word is masked to 2k bits (kmer_shift, :798-805).
+//   ENC_BUILDER: symbol_to_kmer (concurrent_kmer_counter.hpp:2097-2110):
+//                A C T G -> 0 1 2 3, every other byte -> 0; the word is not
+//                masked and keeps the last 4*sizeof(kmer_t) bases.
+constexpr int ENC_MAIN = 0;
+constexpr int ENC_BUILDER = 1;
+
+template <int ENC>
+__device__ __forceinline__ uint32_t base_code(uint32_t c) {
+  if (ENC == ENC_MAIN) return (c >> 1) & 7u;
+  const bool acgt = c == 'A' || c == 'C' || c == 'G' || c == 'T';
+  return acgt ? (c >> 1) & 3u : 0u;
+}
+
 // Calls f(kmer) for every k-mer of bytes [s, s+len), in read order:
-// kmer = ((kmer << 2) | ((c >> 1) & 7)) & mask  (kmer_shift / char_to_el).
-template <class Key, class F>
+// kmer = ((kmer << 2) | code(c)) & mask.
+template <class Key, int ENC, class F>
 __device__ __forceinline__ void for_each_kmer(const uint8_t* fa, uint64_t n, uint64_t s,
                                               uint32_t len, int K, Key mask, F&& f) {
   if (len < (uint32_t)K) return;
@@ -361,7 +428,7 @@ __device__ __forceinline__ void for_each_kmer(const uint8_t* fa, uint64_t n, uin
     } else {
       c = fa[p];
     }
-    km = ((km << 2) | (Key)((c >> 1) & 7u)) & mask;
+    km = ((km << 2) | (Key)base_code<ENC>(c)) & mask;
     if (i + 1 >= (uint32_t)K) f(km);
   }
 }
@@ -382,22 +449,24 @@ struct TileCfg {
   static constexpr int TQ = L1_NT * Q;                 // 4096 (u64) / 2048 (u128): 32 KB
 };
 
-// Level-1 histogram: digit = bits [2K - d1, 2K) of each k-mer.  Block b's
-// count of digit d goes to H[d * nblk + b] (digit-major), so an exclusive
-// scan of H gives every block the output base of each of its digit runs.
-template <class Key>
+// Level-1 histogram: digit = bits [KB - d1, KB) of each k-mer (KB = key
+// bits: 2k on the main path, the whole word for the builder encoding).
+// Block b's count of digit d goes to H[d * nblk + b] (digit-major), so an
+// exclusive scan of H gives every block the output base of each of its
+// digit runs.
+template <class Key, int ENC>
 __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n, const ull* rstart,
-                                                   const uint32_t* rlen, uint64_t R, int K, Key mask,
-                                                   int d1, int rpb, uint32_t* H) {
+                                                   const uint32_t* rlen, uint64_t R, int K, int KB,
+                                                   Key mask, int d1, int rpb, uint32_t* H) {
   __shared__ uint32_t h[256];
   const int nb = 1 << d1;
   for (int i = threadIdx.x; i < nb; i += L1_NT) h[i] = 0;
   __syncthreads();
   const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
   const uint64_t r1 = r0 + rpb < R ? r0 + rpb : R;
-  const int bpos = 2 * K - d1;
+  const int bpos = KB - d1;
   for (uint64_t r = r0 + threadIdx.x; r < r1; r += L1_NT)
-    for_each_kmer<Key>(fa, n, rstart[r], rlen[r], K, mask,
+    for_each_kmer<Key, ENC>(fa, n, rstart[r], rlen[r], K, mask,
                        [&](Key km) { atomicAdd(&h[key_digit<Key>(km, bpos, d1)], 1u); });
   __syncthreads();
   for (int i = threadIdx.x; i < nb; i += L1_NT) H[(ull)i * gridDim.x + blockIdx.x] = h[i];
@@ -450,18 +519,18 @@ __device__ __forceinline__ void tile_split_write(TileLds<Key, NT>& L, int np, Ge
 // (r0 + tid, r0 + tid + L1_NT, ...) with the rolling update and produces up
 // to Q k-mers per tile into registers; the block then splits the tile.  The
 // block's digit bases come from the scanned histogram O (same layout as H).
-template <class Key>
+template <class Key, int ENC>
 __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_t n,
                                                       const ull* rstart, const uint32_t* rlen,
-                                                      uint64_t R, int K, Key mask, int d1, int rpb,
-                                                      const ull* O, Key* S1) {
+                                                      uint64_t R, int K, int KB, Key mask, int d1,
+                                                      int rpb, const ull* O, Key* S1) {
   constexpr int Q = TileCfg<Key>::Q;
   __shared__ TileLds<Key, L1_NT> L;
   const int nb = 1 << d1;
   for (int i = threadIdx.x; i < nb; i += L1_NT) L.base[i] = O[(ull)i * gridDim.x + blockIdx.x];
   const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
   const uint64_t r1 = r0 + rpb < R ? r0 + rpb : R;
-  const int bpos = 2 * K - d1;
+  const int bpos = KB - d1;
   const bool aligned = (reinterpret_cast<uintptr_t>(fa) & 7) == 0;
   uint64_t r = r0 + threadIdx.x;
   uint64_t s = 0;
@@ -502,7 +571,7 @@ __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_
         } else {
           c = fa[p];
         }
-        km = ((km << 2) | (Key)((c >> 1) & 7u)) & mask;
+        km = ((km << 2) | (Key)base_code<ENC>(c)) & mask;
         i++;
         if (i >= (uint32_t)K) {
           kr[j] = km;
@@ -1556,7 +1625,7 @@ struct pk_ctx {
   uint64_t held = 0;
   std::string err;
   DevBuf buf[B_COUNT];
-  hipEvent_t ev[16] = {};
+  hipEvent_t ev[18] = {};
   pk_timing timing{};
   // state of the current table
   int k = 0;
@@ -1564,6 +1633,7 @@ struct pk_ctx {
   uint64_t U = 0, V = 0, M = 0, N = 0, R = 0;
   bool have_table = false, have_kept = false, have_nodes = false;
   int t = 0, mb = 0;
+  int key_bits = 0;  // bits of a table key: 2k (main path) or the word (builder)
 
   int fail(int code, const std::string& what) {
     err = what;
@@ -1801,26 +1871,18 @@ int resolve_overflow(pk_ctx* c, Key* S, Key* R, int32_t* C, int K, int fixed_bit
   return PK_OK;
 }
 
-template <class Key>
-int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
-  constexpr int CAP = HashCfg<Key>::LIMIT;
+// Index of the event bytes ('\n', or '\n' and '>') of fa: *m events at
+// *ev_out.  small[0] receives the position of the first '>'.
+int event_index(pk_ctx* c, const uint8_t* fa, uint64_t n, uint8_t e0, uint8_t e1, ull* small,
+                ull** ev_out, uint64_t* m_out, uint64_t* p0_out) {
   hipStream_t st = c->stream;
-  c->have_table = c->have_kept = c->have_nodes = false;
-  c->timing = pk_timing{};
-  PK_HIP(c, hipEventRecord(c->ev[0], st));
-  // 1. line index
   const uint64_t nb = (n + NL_PER_BLOCK - 1) / NL_PER_BLOCK;
-  ull *blk, *blkoff, *small, *nl, *blkgt;
+  ull *blk, *blkoff, *nl, *blkgt;
   PK_TRY(need(c, B_BLK, nb, &blk));
   PK_TRY(need(c, B_BLKGT, nb + 1, &blkgt));
   PK_TRY(need(c, B_BLKOFF, nb + 1, &blkoff));
-  PK_TRY(need(c, B_SMALL, 16, &small));
-  // small[0] = first '>', [1] = first unprocessed, [2] = first format error,
-  // [3] = first over-long read, [4] = overflow count
-  PK_TRY(fill_u64(c, small, ~0ull, 4));
-  PK_TRY(fill_u64(c, small + 4, 0, 1));
   if (nb) {
-    hipLaunchKernelGGL(k_nl_count, dim3((unsigned)nb), dim3(NL_NT), 0, st, fa, n, blk, blkgt);
+    hipLaunchKernelGGL(k_nl_count, dim3((unsigned)nb), dim3(NL_NT), 0, st, fa, n, e0, e1, blk, blkgt);
     PK_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(1024), 0, st, blkgt, nb, small);
     PK_HIP(c, hipGetLastError());
@@ -1831,13 +1893,30 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
   PK_HIP(c, hipMemcpyAsync(&hm[1], small, 8, hipMemcpyDeviceToHost, st));
   PK_HIP(c, hipStreamSynchronize(st));
   const uint64_t m = hm[0];
-  const uint64_t p0 = hm[1] == ~0ull ? n : hm[1];
   PK_TRY(need(c, B_NL, m + 1, &nl));
   if (nb) {
-    hipLaunchKernelGGL(k_nl_write, dim3((unsigned)nb), dim3(NL_NT), 0, st, fa, n, blkoff, nl);
+    hipLaunchKernelGGL(k_nl_write, dim3((unsigned)nb), dim3(NL_NT), 0, st, fa, n, e0, e1, blkoff, nl);
     PK_HIP(c, hipGetLastError());
   }
-  // 2. read table
+  *ev_out = nl;
+  *m_out = m;
+  *p0_out = hm[1] == ~0ull ? n : hm[1];
+  return PK_OK;
+}
+
+// Read table of the main path: records of the reference's Sliding_window.
+int records_main(pk_ctx* c, const uint8_t* fa, uint64_t n, int K, ull** rs, uint32_t** rl,
+                 uint64_t* R_out, uint64_t* N_out) {
+  hipStream_t st = c->stream;
+  ull* small;
+  PK_TRY(need(c, B_SMALL, 16, &small));
+  // small[0] = first '>', [1] = first unprocessed, [2] = first format error,
+  // [3] = first over-long read, [4] = overflow count
+  PK_TRY(fill_u64(c, small, ~0ull, 4));
+  PK_TRY(fill_u64(c, small + 4, 0, 1));
+  ull* nl;
+  uint64_t m, p0;
+  PK_TRY(event_index(c, fa, n, '\n', '\n', small, &nl, &m, &p0));
   const uint64_t ncand = m / 2 + 2;
   ull *rstart, *kofs;
   uint32_t *rlen, *kc;
@@ -1858,7 +1937,64 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
     return c->fail(PK_EFORMAT, "input is not strict single-line FASTA (record " +
                                    std::to_string(hs[1]) + " does not start with '>')");
   if (hs[2] != ~0ull && hs[2] < R) return c->fail(PK_EINVAL, "read longer than 4 GiB");
-  const uint64_t N = hs[3];
+  *rs = rstart;
+  *rl = rlen;
+  *R_out = R;
+  *N_out = hs[3];
+  return PK_OK;
+}
+
+// Read table of the builder API (bld_record).
+int records_builder(pk_ctx* c, const uint8_t* fa, uint64_t n, int klen, uint64_t read_len, BldSyms sy,
+                    ull** rs, uint32_t** rl, uint64_t* R_out, uint64_t* N_out) {
+  hipStream_t st = c->stream;
+  ull* small;
+  PK_TRY(need(c, B_SMALL, 16, &small));
+  PK_TRY(fill_u64(c, small, ~0ull, 4));
+  PK_TRY(fill_u64(c, small + 4, 0, 1));
+  ull* ev;
+  uint64_t m, p0;
+  PK_TRY(event_index(c, fa, n, sy.delim, sy.skip, small, &ev, &m, &p0));
+  ull *rstart, *kofs, *idx;
+  uint32_t *rlen, *kc, *flag;
+  PK_TRY(need(c, B_RSTART, m + 1, &rstart));
+  PK_TRY(need(c, B_RLEN, m + 1, &rlen));
+  PK_TRY(need(c, B_KC, m + 1, &kc));
+  PK_TRY(need(c, B_KOFS, m + 2, &kofs));
+  PK_TRY(need(c, B_TU, m + 1, &flag));
+  PK_TRY(need(c, B_UOFF, m + 2, &idx));
+  if (m) {
+    hipLaunchKernelGGL(k_bld_flags, dim3(grid_for(m, 256)), dim3(256), 0, st, fa, n, ev, m, klen,
+                       read_len, sy, flag);
+    PK_HIP(c, hipGetLastError());
+  }
+  PK_TRY(scan_excl<uint32_t>(c, flag, idx, m));
+  ull R = 0;
+  PK_TRY(d2h(c, &R, idx + m, 8));
+  if (m) {
+    hipLaunchKernelGGL(k_bld_write, dim3(grid_for(m, 256)), dim3(256), 0, st, fa, n, ev, m, klen,
+                       read_len, sy, idx, rstart, rlen, kc);
+    PK_HIP(c, hipGetLastError());
+  }
+  PK_TRY(scan_excl<uint32_t>(c, kc, kofs, R));
+  ull N = 0;
+  PK_TRY(d2h(c, &N, kofs + R, 8));
+  *rs = rstart;
+  *rl = rlen;
+  *R_out = R;
+  *N_out = N;
+  return PK_OK;
+}
+
+// Count the k-mers of the read table (rstart, rlen)[0..R), N instances.
+// K = window length, KB = key bits (2K on the main path, the whole word for
+// the builder encoding), mask = word mask of the rolling update.
+template <class Key, int ENC>
+int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, const uint32_t* rlen,
+                  uint64_t R, uint64_t N, int K, int KB, Key mask) {
+  constexpr int CAP = HashCfg<Key>::LIMIT;
+  hipStream_t st = c->stream;
+  ull* small = reinterpret_cast<ull*>(c->buf[B_SMALL].p);
   c->R = R;
   c->N = N;
   c->k = K;
@@ -1878,8 +2014,7 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
   // 3. partition: level 1 fused with the parse, further levels from memory,
   // at most 8 bits per level, until a bucket holds ~1.5 hash tables' worth
   // of instances.  Write bases come from scanned digit-major histograms.
-  const Key mask = key_mask<Key>(2 * K);
-  const int btot = choose_bits(N, K, CAP);
+  const int btot = choose_bits(N, KB / 2, CAP);
   const int d1 = btot < 8 ? btot : 8;
   const uint64_t B1 = 1ull << d1;
   const int rpb = 2048;  // reads per block
@@ -1895,8 +2030,8 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
   PK_TRY(need(c, B_C, N, &C));
   c->timing.bucket_bits = btot;
   PK_HIP(c, hipEventRecord(c->ev[2], st));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_hist<Key>), dim3((unsigned)nblk), dim3(L1_NT), 0, st, fa, n,
-                     rstart, rlen, R, K, mask, d1, rpb, H);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_hist<Key, ENC>), dim3((unsigned)nblk), dim3(L1_NT), 0, st, fa,
+                     n, rstart, rlen, R, K, KB, mask, d1, rpb, H);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[3], st));
   PK_TRY(scan_excl<uint32_t>(c, H, O, B1 * nblk));
@@ -1904,9 +2039,10 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
                      boff);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[4], st));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key>), dim3((unsigned)nblk), dim3(L1_NT), 0, st, fa,
-                     n, rstart, rlen, R, K, mask, d1, rpb, O, S);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key, ENC>), dim3((unsigned)nblk), dim3(L1_NT), 0, st,
+                     fa, n, rstart, rlen, R, K, KB, mask, d1, rpb, O, S);
   PK_HIP(c, hipGetLastError());
+  PK_HIP(c, hipEventRecord(c->ev[16], st));
   uint64_t B = B1;
   int fixed = d1;
   int set = 0;
@@ -1930,7 +2066,7 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
     PK_HIP(c, hipGetLastError());
     ull ntiles = 0;
     PK_TRY(d2h(c, &ntiles, tp + B, 8));
-    const int dpos = 2 * K - fixed - db;
+    const int dpos = KB - fixed - db;
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_hist<Key>), dim3((unsigned)ntiles), dim3(MS_NT), 0, st, Sin,
                        boff, tp, (int)B, dpos, db, H2);
     PK_HIP(c, hipGetLastError());
@@ -1963,7 +2099,7 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
   PK_TRY(need(c, B_UOFF, B + 1, &uoff));
   PK_HIP(c, hipEventRecord(c->ev[6], st));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bucket_hash<Key>), dim3((unsigned)B), dim3(BS_NT), 0, st, S,
-                     Res, C, boff, bcnt, B, K, btot, tu, small + 4, ovl);
+                     Res, C, boff, bcnt, B, KB / 2, btot, tu, small + 4, ovl);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[7], st));
   ull novf = 0;
@@ -1989,7 +2125,7 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
         ob.push_back(b);
       }
     std::vector<std::vector<Leaf>> rec;
-    PK_TRY(resolve_overflow<Key>(c, S, Res, C, K, btot, os, oc, rec));
+    PK_TRY(resolve_overflow<Key>(c, S, Res, C, KB / 2, btot, os, oc, rec));
     std::vector<ull> ls;
     std::vector<long long> lus;
     size_t j = 0;
@@ -2037,11 +2173,40 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
   c->timing.parse_ms = el(0, 1);
   c->timing.hist_ms = el(2, 3);
   c->timing.scatter_ms = el(4, 5);
+  c->timing.l1_scatter_ms = el(4, 16);
   c->timing.sort_ms = el(6, 7);
   c->timing.split_ms = novf ? el(8, 9) : 0.f;
   c->timing.compact_ms = el(10, 11);
   c->have_table = true;
   return PK_OK;
+}
+
+template <class Key>
+int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
+  c->have_table = c->have_kept = c->have_nodes = false;
+  c->timing = pk_timing{};
+  c->key_bits = 2 * K;
+  PK_HIP(c, hipEventRecord(c->ev[0], c->stream));
+  ull* rstart;
+  uint32_t* rlen;
+  uint64_t R, N;
+  PK_TRY(records_main(c, fa, n, K, &rstart, &rlen, &R, &N));
+  return count_records<Key, ENC_MAIN>(c, fa, n, rstart, rlen, R, N, K, 2 * K, key_mask<Key>(2 * K));
+}
+
+template <class Key>
+int builder_count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int klen, uint64_t read_len,
+                       BldSyms sy) {
+  c->have_table = c->have_kept = c->have_nodes = false;
+  c->timing = pk_timing{};
+  c->key_bits = (int)sizeof(Key) * 8;
+  PK_HIP(c, hipEventRecord(c->ev[0], c->stream));
+  ull* rstart;
+  uint32_t* rlen;
+  uint64_t R, N;
+  PK_TRY(records_builder(c, fa, n, klen, read_len, sy, &rstart, &rlen, &R, &N));
+  return count_records<Key, ENC_BUILDER>(c, fa, n, rstart, rlen, R, N, klen, (int)sizeof(Key) * 8,
+                                         ~Key(0));
 }
 
 template <class Key>
@@ -2242,6 +2407,37 @@ int pk_count(pk_ctx* c, const char* fasta, uint64_t nbytes, int k, int fasta_on_
   return PK_OK;
 }
 
+int pk_builder_count(pk_ctx* c, const char* fasta, uint64_t nbytes, int klen, uint64_t read_len,
+                     int kmer_bits, char delimiter, char skip_symbol, int fasta_on_device,
+                     pk_table* out, uint64_t* n_reads) {
+  if (!c || !out || (!fasta && nbytes)) return PK_EINVAL;
+  if (kmer_bits != 64 && kmer_bits != 128) return c->fail(PK_EINVAL, "kmer_bits must be 64 or 128");
+  if (klen < 1) return c->fail(PK_EINVAL, "klen must be >= 1");
+  if (read_len > 0xFFFFFFF0ull) return c->fail(PK_EINVAL, "read_len must be < 2^32 - 16");
+  if (delimiter == skip_symbol) return c->fail(PK_EINVAL, "delimiter and skip_symbol must differ");
+  const BldSyms sy{(uint8_t)delimiter, (uint8_t)skip_symbol};
+  PK_HIP(c, hipSetDevice(c->device));
+  const uint8_t* fa;
+  if (fasta_on_device) {
+    fa = reinterpret_cast<const uint8_t*>(fasta);
+  } else {
+    uint8_t* d;
+    PK_TRY(need(c, B_FASTA, nbytes + 64, &d));
+    if (nbytes) PK_HIP(c, hipMemcpyAsync(d, fasta, nbytes, hipMemcpyHostToDevice, c->stream));
+    fa = d;
+  }
+  int rc = kmer_bits == 64 ? builder_count_impl<uint64_t>(c, fa, nbytes, klen, read_len, sy)
+                           : builder_count_impl<u128>(c, fa, nbytes, klen, read_len, sy);
+  if (rc) return rc;
+  out->n = c->U;
+  out->words = c->words;
+  out->k = c->k;
+  out->keys = reinterpret_cast<const uint64_t*>(c->buf[B_TK].p);
+  out->counts = reinterpret_cast<const int32_t*>(c->buf[B_TC].p);
+  if (n_reads) *n_reads = c->R;
+  return PK_OK;
+}
+
 int pk_cutoff(pk_ctx* c, int t, uint64_t* hist_t, int* min_bucket, uint64_t* n_kept) {
   if (!c || !min_bucket || !n_kept) return PK_EINVAL;
   if (!c->have_table) return c->fail(PK_ESTATE, "pk_cutoff before pk_count");
@@ -2266,6 +2462,9 @@ int pk_kept_table(pk_ctx* c, pk_table* out) {
 int pk_macro_nodes(pk_ctx* c, pk_mn_table* out) {
   if (!c || !out) return PK_EINVAL;
   if (!c->have_kept || c->V == 0) return c->fail(PK_ESTATE, "pk_macro_nodes needs a non-empty kept table");
+  if (c->key_bits != 2 * c->k)
+    return c->fail(PK_ESTATE, "macro nodes need k-mer keys of exactly 2k bits (builder tables with "
+                              "klen != 4 * sizeof(kmer_t) keep extra bases)");
   PK_HIP(c, hipSetDevice(c->device));
   int rc = c->words == 1 ? mn_impl<uint64_t>(c) : mn_impl<u128>(c);
   if (rc) return rc;

@@ -1,0 +1,80 @@
+// Drives pakhip::Hip_Kmer_Partial_Counter the way the reference's builder
+// callers drive Concurrent_Kmer_Partial_Counter (src/default_kmer_table_builder.cpp:
+// 16-81, src/concurrent_kmer_counter_test.cpp:82-114): an aggregating
+// processor with atomic counters, one call of default_count_and_process.
+//
+//   builder_adapter_test                       reference unit-test input, klen 32
+//   builder_adapter_test FILE KLEN READ_LEN BITS MAX_KMER_COUNT
+//
+// Prints one line "reads=R unique=U freq=F calls=C xor=X" (X = xor of
+// key words and counts over all pairs seen, an order-independent digest).
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "pakhip_builders.hpp"
+
+struct count_unique_kmers_t {
+  count_unique_kmers_t() : count_(0UL), kmer_freq_(0UL), calls_(0UL), digest_(0UL) {}
+  template <typename iterator>
+  void operator()(iterator beg, iterator end) const {
+    ++calls_;
+    for (; beg != end; ++beg) {
+      ++count_;
+      kmer_freq_ += beg->second;
+      const unsigned __int128 w = (unsigned __int128)beg->first;
+      digest_ ^= (unsigned long)(w ^ (w >> 64)) * 0x9E3779B97F4A7C15ull + (unsigned long)beg->second;
+    }
+  }
+  mutable std::atomic<size_t> count_;
+  mutable std::atomic<size_t> kmer_freq_;
+  mutable std::atomic<size_t> calls_;
+  mutable std::atomic<unsigned long> digest_;
+};
+
+template <class kmer_t>
+int run(const std::string& input, size_t klen, size_t read_len, size_t max_kmer_count) {
+  pakhip::Hip_Kmer_Partial_Counter<kmer_t> counter(klen, input.data(), input.data() + input.size(),
+                                                   read_len, max_kmer_count);
+  count_unique_kmers_t p;
+  const size_t reads = counter.default_count_and_process(p);
+  printf("reads=%zu unique=%zu freq=%zu calls=%zu xor=%lu\n", reads, (size_t)p.count_,
+         (size_t)p.kmer_freq_, (size_t)p.calls_, (unsigned long)p.digest_);
+  return 0;
+}
+
+int main(int argc, char** argv) {
+  try {
+    if (argc < 2) {
+      // src/concurrent_kmer_counter_test.cpp:121-126
+      const std::string input =
+          ">CACVBX020000001.1-NC_000913.3520\n"
+          "TACCATTCACCCGCTGGCGGGGCAGGGGGTAAATCTCGGCTTTATGGATGCTGCAGAGCTGATTGCCGAACTGAAACGGTTGCATCGTCAGGGGAAAGAC\n"
+          ">CACVBX020000001.1-NC_000913.3519\n"
+          "TGGTTTGGCCGTCGCTTAGGCCAGGATGCCGCACCAGAGAAACTGGGCGTTGACCGCGCACTGGCATTCAGAGAAATCAATCAGCAACTTTATCAACTAC\n";
+      return run<size_t>(input, 32, 100, std::numeric_limits<size_t>::max());
+    }
+    if (argc < 6) {
+      fprintf(stderr, "usage: %s FILE KLEN READ_LEN BITS MAX_KMER_COUNT\n", argv[0]);
+      return 2;
+    }
+    FILE* f = fopen(argv[1], "rb");
+    if (!f) throw std::logic_error(std::string("Unable to open file: ") + argv[1]);
+    std::string input;
+    char buf[1 << 16];
+    size_t r;
+    while ((r = fread(buf, 1, sizeof buf, f)) > 0) input.append(buf, r);
+    fclose(f);
+    const size_t klen = strtoull(argv[2], nullptr, 10), read_len = strtoull(argv[3], nullptr, 10);
+    const int bits = atoi(argv[4]);
+    const size_t maxc = strtoull(argv[5], nullptr, 10);
+    return bits == 64 ? run<size_t>(input, klen, read_len, maxc)
+                      : run<unsigned __int128>(input, klen, read_len, maxc);
+  } catch (const std::logic_error& e) {
+    printf("logic_error: %s\n", e.what());
+    return 3;
+  }
+}

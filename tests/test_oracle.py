@@ -222,3 +222,47 @@ def test_reference_pipeline_detects_planted_mistakes(oracle_mod):
     # the reference's cutoff abort case (all k-mers unique)
     two = read_ref("ref_test_two_reads_k32.ref.txt")
     assert two.aborted and two.min_bucket == 2 and two.kept == 0 and two.distinct == 138
+
+
+# ---------------------------------------------------------------------------
+# Builder API (concurrent_kmer_counter.hpp:2018-2082): the oracle against an
+# independent closed-form restatement (tests/builder_util.py).
+
+from builder_util import builder_closed_form, edge_inputs, table_to_counter  # noqa: E402
+
+BUILDER_PARAMS = [(1, 100, 64), (5, 30, 64), (21, 100, 64), (32, 100, 64), (33, 100, 64),
+                  (32, 100, 128), (40, 60, 128), (64, 100, 128)]
+
+
+@pytest.mark.parametrize("name", sorted(edge_inputs()))
+@pytest.mark.parametrize("klen,read_len,bits", BUILDER_PARAMS)
+def test_builder_oracle_matches_closed_form(oracle_mod, name, klen, read_len, bits):
+    fa = edge_inputs()[name]
+    reads, keys, counts = oracle_mod.builder_count(fa, klen, read_len, bits)
+    want_reads, want = builder_closed_form(fa, klen, read_len, bits)
+    assert reads == want_reads
+    assert table_to_counter(keys, counts) == want
+    if keys.shape[0]:
+        assert keys.shape[1] == bits // 64
+
+
+def test_builder_oracle_custom_symbols(oracle_mod):
+    fa = edge_inputs()["ragged"].replace(b"\n", b"|").replace(b">", b"@")
+    reads, keys, counts = oracle_mod.builder_count(fa, 21, 100, 64, b"|", b"@")
+    want_reads, want = builder_closed_form(fa, 21, 100, 64, b"|", b"@")
+    assert reads == want_reads and reads > 0
+    assert table_to_counter(keys, counts) == want
+
+
+def test_builder_closed_form_detects_planted_mistakes():
+    """The restatement is sensitive to the rules it checks: a wrong base code,
+    a dropped short-read rule or a wrong record rule change its result."""
+    fa = edge_inputs()["junk_and_multi_gt"]
+    reads, want = builder_closed_form(fa, 21, 100, 64)
+    _, alt = builder_closed_form(fa.replace(b"G", b"T"), 21, 100, 64)
+    assert alt != want
+    # reads shorter than klen are skipped: read_len 20 < klen 21 leaves none
+    assert builder_closed_form(fa, 21, 20, 64)[0] == 0 and builder_closed_form(fa, 20, 20, 64)[0] > 0
+    # the first '>' of a sequence line starts a record
+    r3, _ = builder_closed_form(fa.replace(b"seq>with>gt", b"seq with gt"), 21, 100, 64)
+    assert r3 == reads - 1
