@@ -40,6 +40,7 @@ extern "C" {
 #define PK_EEMPTY (-5)  /* no k-mer survives the cutoff (the reference calls
                            MPI_Abort, src/kmer_histogram.cpp:189-200) */
 #define PK_ESTATE (-6)  /* call out of order (e.g. cutoff before count) */
+#define PK_EAGAIN (-7)  /* pk_server_poll: the job has not finished */
 
 typedef struct pk_ctx pk_ctx;
 
@@ -101,6 +102,9 @@ const char* pk_last_error(const pk_ctx* ctx);
 void* pk_ctx_stream(pk_ctx* ctx);
 /* Device bytes currently held by the context. */
 uint64_t pk_ctx_device_bytes(const pk_ctx* ctx);
+/* Free every device buffer of the context (they grow back on the next call);
+ * invalidates the table, kept table and node pointers. */
+int pk_ctx_release_buffers(pk_ctx* ctx);
 
 /* Count the k-mers of a strict single-line FASTA buffer (main path of the
  * reference, kmer_counting.cpp:2890-3261).  fasta may be a host pointer
@@ -148,6 +152,64 @@ int pk_download_kept(pk_ctx* ctx, uint64_t* keys, int32_t* counts);
 int pk_download_nodes(pk_ctx* ctx, pk_mnode* nodes);
 
 int pk_last_timing(const pk_ctx* ctx, pk_timing* out);
+
+/* ---- Batched and served forms (no single counterpart in the reference: many
+ * independent read sets, each with the pk_count / pk_cutoff / pk_macro_nodes
+ * semantics, overlapped on one device) ----
+ *
+ * A server owns n_slots contexts on one device, each driven by its own host
+ * thread and HIP stream, so that the jobs' kernels overlap on the GPU.  Jobs
+ * wait in a FIFO queue and start when a slot is free and their estimated HBM
+ * footprint, added to the running jobs' estimates and the resident results,
+ * fits the budget (a job always starts when nothing runs).  A job's results
+ * (table, kept table, macro nodes) stay resident in device memory owned by
+ * the server until pk_server_release.  Results are identical to the same job
+ * run alone on a context.  All pk_server_* calls are thread-safe. */
+typedef struct pk_job {
+  const char* fasta;       /* host or device bytes, valid until the job ends */
+  uint64_t nbytes;
+  int32_t k;               /* 8 <= k <= 64 */
+  int32_t t;               /* cutoff buckets as pk_cutoff (0 -> 21); < 0: count only */
+  int32_t fasta_on_device; /* fasta is a device pointer */
+  int32_t want_nodes;      /* also build macro nodes (needs t >= 0) */
+} pk_job;
+
+typedef struct pk_job_result {
+  int32_t status;          /* PK_OK, or the job's PK_E* code (PK_EEMPTY keeps the table) */
+  int32_t min_bucket;
+  uint64_t n_reads;
+  uint64_t n_instances;
+  pk_table table;          /* device pointers owned by the server */
+  pk_table kept;
+  pk_mn_table nodes;
+  double t_submit, t_start, t_end; /* seconds on the pk_server_now() clock */
+  uint64_t est_bytes;      /* admission estimate of the job's working set */
+  char message[160];       /* error message when status != PK_OK */
+} pk_job_result;
+
+typedef struct pk_server pk_server;
+
+/* hbm_budget_bytes = 0: 90 % of the device memory free at creation. */
+int pk_server_create(pk_server** out, int device, int n_slots, uint64_t hbm_budget_bytes);
+/* Drops queued jobs, waits for running ones, frees everything. */
+void pk_server_destroy(pk_server* srv);
+double pk_server_now(const pk_server* srv);
+int pk_server_submit(pk_server* srv, const pk_job* job, uint64_t* ticket);
+/* Enqueue n jobs at once, largest first (longest-processing-time order);
+ * tickets[i] belongs to jobs[i]. */
+int pk_server_submit_batch(pk_server* srv, const pk_job* jobs, int n, uint64_t* tickets);
+/* Block until the job has finished; fill *out. */
+int pk_server_wait(pk_server* srv, uint64_t ticket, pk_job_result* out);
+/* PK_EAGAIN while the job is queued or running, else as pk_server_wait. */
+int pk_server_poll(pk_server* srv, uint64_t ticket, pk_job_result* out);
+/* Copy a finished job's results to host arrays (any pointer may be NULL):
+ * table keys/counts, kept keys/counts, hist_t (t entries), nodes. */
+int pk_server_download(pk_server* srv, uint64_t ticket, uint64_t* keys, int32_t* counts,
+                       uint64_t* kept_keys, int32_t* kept_counts, uint64_t* hist_t,
+                       pk_mnode* nodes);
+/* Free a finished job's device results and forget the ticket. */
+int pk_server_release(pk_server* srv, uint64_t ticket);
+const char* pk_server_last_error(const pk_server* srv);
 
 #ifdef __cplusplus
 }

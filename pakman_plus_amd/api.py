@@ -20,7 +20,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-PK_OK, PK_EINVAL, PK_ENOMEM, PK_EHIP, PK_EFORMAT, PK_EEMPTY, PK_ESTATE = 0, -1, -2, -3, -4, -5, -6
+PK_OK, PK_EINVAL, PK_ENOMEM, PK_EHIP, PK_EFORMAT, PK_EEMPTY, PK_ESTATE, PK_EAGAIN = 0, -1, -2, -3, -4, -5, -6, -7
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libpakhip.so")
 
@@ -50,14 +50,31 @@ class _Timing(ctypes.Structure):
                 ("l1_scatter_ms", ctypes.c_float)]
 
 
+class _Job(ctypes.Structure):
+    _fields_ = [("fasta", ctypes.c_void_p), ("nbytes", ctypes.c_uint64), ("k", ctypes.c_int32),
+                ("t", ctypes.c_int32), ("fasta_on_device", ctypes.c_int32),
+                ("want_nodes", ctypes.c_int32)]
+
+
+class _JobResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("min_bucket", ctypes.c_int32),
+                ("n_reads", ctypes.c_uint64), ("n_instances", ctypes.c_uint64),
+                ("table", _Table), ("kept", _Table), ("nodes", _MnTable),
+                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double),
+                ("t_end", ctypes.c_double), ("est_bytes", ctypes.c_uint64),
+                ("message", ctypes.c_char * 160)]
+
+
 MNODE_DTYPE = np.dtype([("key_lo", "<u8"), ("key_hi", "<u8"), ("pre_cnt", "<i4", (4,)),
                         ("suf_cnt", "<i4", (4,))])
 
 # every symbol declared in include/pakhip.h
 EXPORTS = ("pk_ctx_create", "pk_ctx_destroy", "pk_last_error", "pk_ctx_stream",
-           "pk_ctx_device_bytes", "pk_count", "pk_builder_count", "pk_cutoff", "pk_kept_table",
-           "pk_macro_nodes", "pk_download_table", "pk_download_kept", "pk_download_nodes",
-           "pk_last_timing")
+           "pk_ctx_device_bytes", "pk_ctx_release_buffers", "pk_count", "pk_builder_count",
+           "pk_cutoff", "pk_kept_table", "pk_macro_nodes", "pk_download_table", "pk_download_kept",
+           "pk_download_nodes", "pk_last_timing", "pk_server_create", "pk_server_destroy",
+           "pk_server_now", "pk_server_submit", "pk_server_submit_batch", "pk_server_wait",
+           "pk_server_poll", "pk_server_download", "pk_server_release", "pk_server_last_error")
 
 _LIB = None
 
@@ -107,6 +124,28 @@ def load_library():
     L.pk_download_nodes.restype = i32
     L.pk_last_timing.argtypes = [vp, ctypes.POINTER(_Timing)]
     L.pk_last_timing.restype = i32
+    L.pk_ctx_release_buffers.argtypes = [vp]
+    L.pk_ctx_release_buffers.restype = i32
+    L.pk_server_create.argtypes = [ctypes.POINTER(vp), i32, i32, u64]
+    L.pk_server_create.restype = i32
+    L.pk_server_destroy.argtypes = [vp]
+    L.pk_server_destroy.restype = None
+    L.pk_server_now.argtypes = [vp]
+    L.pk_server_now.restype = ctypes.c_double
+    L.pk_server_submit.argtypes = [vp, ctypes.POINTER(_Job), ctypes.POINTER(u64)]
+    L.pk_server_submit.restype = i32
+    L.pk_server_submit_batch.argtypes = [vp, ctypes.POINTER(_Job), i32, ctypes.POINTER(u64)]
+    L.pk_server_submit_batch.restype = i32
+    L.pk_server_wait.argtypes = [vp, u64, ctypes.POINTER(_JobResult)]
+    L.pk_server_wait.restype = i32
+    L.pk_server_poll.argtypes = [vp, u64, ctypes.POINTER(_JobResult)]
+    L.pk_server_poll.restype = i32
+    L.pk_server_download.argtypes = [vp, u64, vp, vp, vp, vp, vp, vp]
+    L.pk_server_download.restype = i32
+    L.pk_server_release.argtypes = [vp, u64]
+    L.pk_server_release.restype = i32
+    L.pk_server_last_error.argtypes = [vp]
+    L.pk_server_last_error.restype = ctypes.c_char_p
     _LIB = L
     return L
 
@@ -256,6 +295,147 @@ class Context:
         t = _Timing()
         self._check(self._lib.pk_last_timing(self._h, ctypes.byref(t)))
         return {f: getattr(t, f) for f, _ in _Timing._fields_}
+
+
+# ---------------------------------------------------------------------------
+# Batched / served forms (pk_server_*).
+
+@dataclass
+class JobResult:
+    ticket: int
+    status: int
+    min_bucket: int
+    n_reads: int
+    n_instances: int
+    n_unique: int
+    n_kept: int
+    n_nodes: int
+    words: int
+    t: int
+    t_submit: float
+    t_start: float
+    t_end: float
+    est_bytes: int
+    message: str
+
+
+class Server:
+    """n_slots contexts on one device serving count [+ cutoff [+ macro nodes]]
+    jobs; results stay resident on the device until release()."""
+
+    def __init__(self, device: int = 0, n_slots: int = 4, hbm_budget_bytes: int = 0):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self._lib.pk_server_create(ctypes.byref(h), device, n_slots, hbm_budget_bytes)
+        if rc:
+            raise PakHipError(rc, "pk_server_create failed")
+        self._h = h
+        self._keep = {}   # ticket -> host buffers that must outlive the job
+        self._t = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pk_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise PakHipError(rc, self._lib.pk_server_last_error(self._h).decode(errors="replace"))
+
+    def now(self) -> float:
+        return float(self._lib.pk_server_now(self._h))
+
+    @staticmethod
+    def _job(fasta, k, t, want_nodes, device_ptr, nbytes):
+        j = _Job()
+        keep = None
+        if device_ptr is not None:
+            j.fasta, j.nbytes, j.fasta_on_device = device_ptr, nbytes, 1
+        else:
+            keep = _as_u8(fasta)
+            j.fasta, j.nbytes, j.fasta_on_device = keep.ctypes.data, keep.nbytes, 0
+        j.k, j.t, j.want_nodes = k, t, 1 if want_nodes else 0
+        return j, keep
+
+    def submit(self, fasta=None, k: int = 32, t: int = 21, want_nodes: bool = True,
+               device_ptr: int | None = None, nbytes: int | None = None) -> int:
+        j, keep = self._job(fasta, k, t, want_nodes, device_ptr, nbytes)
+        tk = ctypes.c_uint64()
+        self._check(self._lib.pk_server_submit(self._h, ctypes.byref(j), ctypes.byref(tk)))
+        self._keep[tk.value] = keep
+        self._t[tk.value] = 21 if t == 0 else t
+        return int(tk.value)
+
+    def submit_batch(self, jobs: list[dict]) -> list[int]:
+        """jobs: dicts with the keyword arguments of submit().  Enqueued
+        largest first; returns the tickets in the order given."""
+        n = len(jobs)
+        arr = (_Job * n)()
+        keeps = []
+        for i, d in enumerate(jobs):
+            j, keep = self._job(d.get("fasta"), d.get("k", 32), d.get("t", 21), d.get("want_nodes", True),
+                                d.get("device_ptr"), d.get("nbytes"))
+            arr[i] = j
+            keeps.append(keep)
+        tks = (ctypes.c_uint64 * n)()
+        self._check(self._lib.pk_server_submit_batch(self._h, arr, n, tks))
+        out = [int(x) for x in tks]
+        for tk, keep, d in zip(out, keeps, jobs):
+            self._keep[tk] = keep
+            t = d.get("t", 21)
+            self._t[tk] = 21 if t == 0 else t
+        return out
+
+    def _result(self, tk, r) -> JobResult:
+        return JobResult(tk, r.status, r.min_bucket, r.n_reads, r.n_instances, r.table.n, r.kept.n,
+                         r.nodes.n, r.table.words, self._t.get(tk, 0), r.t_submit, r.t_start, r.t_end,
+                         r.est_bytes, r.message.decode(errors="replace"))
+
+    def wait(self, ticket: int) -> JobResult:
+        r = _JobResult()
+        self._check(self._lib.pk_server_wait(self._h, ticket, ctypes.byref(r)))
+        self._keep.pop(ticket, None)
+        return self._result(ticket, r)
+
+    def poll(self, ticket: int) -> JobResult | None:
+        r = _JobResult()
+        rc = self._lib.pk_server_poll(self._h, ticket, ctypes.byref(r))
+        if rc == PK_EAGAIN:
+            return None
+        self._check(rc)
+        self._keep.pop(ticket, None)
+        return self._result(ticket, r)
+
+    def download(self, res: JobResult) -> dict:
+        w = max(res.words, 1)
+        keys = np.empty((res.n_unique, w), dtype=np.uint64)
+        counts = np.empty(res.n_unique, dtype=np.int32)
+        kk = np.empty((res.n_kept, w), dtype=np.uint64)
+        kc = np.empty(res.n_kept, dtype=np.int32)
+        hist = np.zeros(max(res.t, 1), dtype=np.uint64)
+        nodes = np.empty(res.n_nodes, dtype=MNODE_DTYPE)
+        self._check(self._lib.pk_server_download(self._h, res.ticket, keys.ctypes.data, counts.ctypes.data,
+                                                 kk.ctypes.data, kc.ctypes.data, hist.ctypes.data,
+                                                 nodes.ctypes.data))
+        return {"keys": keys, "counts": counts, "kept_keys": kk, "kept_counts": kc, "hist": hist,
+                "nodes": nodes}
+
+    def release(self, ticket: int):
+        self._check(self._lib.pk_server_release(self._h, ticket))
+        self._t.pop(ticket, None)
 
 
 def nodes_as_arrays(nodes: np.ndarray, k: int) -> dict:

@@ -29,7 +29,7 @@ HIP_FLAGS = [
     "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function",
 ]
 
-HIP_SOURCES = ["pk_device.hip"]
+HIP_SOURCES = ["pk_device.hip", "pk_server.cpp"]
 
 
 def _run(cmd):

@@ -2383,6 +2383,20 @@ const char* pk_last_error(const pk_ctx* c) { return c ? c->err.c_str() : "null c
 void* pk_ctx_stream(pk_ctx* c) { return c ? (void*)c->stream : nullptr; }
 uint64_t pk_ctx_device_bytes(const pk_ctx* c) { return c ? c->held : 0; }
 
+int pk_ctx_release_buffers(pk_ctx* c) {
+  if (!c) return PK_EINVAL;
+  PK_HIP(c, hipSetDevice(c->device));
+  PK_HIP(c, hipStreamSynchronize(c->stream));
+  for (auto& b : c->buf) {
+    if (b.p) PK_HIP(c, hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  c->held = 0;
+  c->have_table = c->have_kept = c->have_nodes = false;
+  return PK_OK;
+}
+
 int pk_count(pk_ctx* c, const char* fasta, uint64_t nbytes, int k, int fasta_on_device,
              pk_table* out) {
   if (!c || !out || (!fasta && nbytes)) return PK_EINVAL;

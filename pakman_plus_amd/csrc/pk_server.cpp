@@ -1,0 +1,384 @@
+// pk_server.cpp -- batched and served forms of the counting / cutoff /
+// macro-node path (pk_server_* in include/pakhip.h).
+//
+// n_slots contexts on one device; slot i is driven by host thread i on the
+// context's own HIP stream, so jobs in different slots overlap on the GPU
+// (their host synchronisations, small launches and tails fill each other's
+// gaps).  Jobs are admitted FIFO by estimated HBM footprint.  A finished
+// job's table, kept table and nodes are copied into one device allocation
+// owned by the job (stream-ordered allocator when the device supports memory
+// pools, so steady-state serving does not synchronise the device) and stay
+// resident until pk_server_release.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "pakhip.h"
+
+namespace {
+
+struct Job {
+  pk_job spec{};
+  uint64_t ticket = 0;
+  int state = 0;  // 0 queued, 1 running, 2 done
+  pk_job_result res{};
+  std::vector<uint64_t> hist;
+  void* mem = nullptr;
+  uint64_t mem_bytes = 0;
+};
+
+inline uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
+
+// Working-set estimate of a job: FASTA copy (host input), line index and read
+// table, two instance arrays and the per-instance counts, the table.  The
+// instance count is bounded by the byte count.
+uint64_t estimate_bytes(const pk_job& j) {
+  const uint64_t W = j.k <= 32 ? 8 : 16;
+  const uint64_t n = j.nbytes;
+  uint64_t b = n * (2 * W + 4) + n * (W + 4) + n * 2 + (64ull << 20);
+  if (!j.fasta_on_device) b += n;
+  return b;
+}
+
+}  // namespace
+
+struct pk_server {
+  int device = 0;
+  int n_slots = 0;
+  uint64_t budget = 0;
+  bool pools = false;
+  std::vector<pk_ctx*> ctx;
+  std::vector<std::thread> workers;
+  hipStream_t free_stream = nullptr;
+  std::mutex mu;
+  std::condition_variable cv_q, cv_done;
+  std::deque<Job*> queue;
+  std::unordered_map<uint64_t, Job*> jobs;
+  uint64_t next_ticket = 1;
+  uint64_t running_est = 0, resident = 0;
+  int running = 0;
+  bool stop = false;
+  std::string err;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  bool admissible() const {
+    if (queue.empty()) return false;
+    if (running == 0) return true;
+    return running_est + resident + queue.front()->res.est_bytes <= budget;
+  }
+  int fail(int code, const std::string& m) {
+    err = m;
+    return code;
+  }
+  void worker(int slot);
+  void run(int slot, Job* j);
+};
+
+void pk_server::run(int slot, Job* j) {
+  pk_ctx* c = ctx[slot];
+  pk_job_result& r = j->res;
+  auto set_err = [&](int rc) {
+    r.status = rc;
+    std::snprintf(r.message, sizeof(r.message), "%s", pk_last_error(c));
+  };
+  pk_table t{};
+  int rc = pk_count(c, j->spec.fasta, j->spec.nbytes, j->spec.k, j->spec.fasta_on_device, &t);
+  pk_timing tm{};
+  pk_last_timing(c, &tm);
+  r.n_reads = tm.n_reads;
+  r.n_instances = tm.n_instances;
+  if (rc) {
+    set_err(rc);
+    return;
+  }
+  pk_table kt{};
+  pk_mn_table mt{};
+  int crc = PK_OK;
+  if (j->spec.t >= 0) {
+    const int tt = j->spec.t == 0 ? 21 : j->spec.t;
+    j->hist.assign((size_t)tt, 0);
+    uint64_t nk = 0;
+    int mb = 0;
+    crc = pk_cutoff(c, j->spec.t, j->hist.data(), &mb, &nk);
+    r.min_bucket = mb;
+    if (crc == PK_OK) {
+      pk_kept_table(c, &kt);
+      if (j->spec.want_nodes) crc = pk_macro_nodes(c, &mt);
+    }
+    if (crc) set_err(crc);
+    if (crc && crc != PK_EEMPTY) return;
+  }
+  // one allocation: table keys | counts | kept keys | counts | nodes
+  const uint64_t w = (uint64_t)t.words * 8;
+  const uint64_t o1 = 0, o2 = align256(o1 + t.n * w), o3 = align256(o2 + t.n * 4),
+                 o4 = align256(o3 + kt.n * w), o5 = align256(o4 + kt.n * 4),
+                 end = align256(o5 + mt.n * sizeof(pk_mnode));
+  hipStream_t st = (hipStream_t)pk_ctx_stream(c);
+  void* mem = nullptr;
+  hipError_t e = pools ? hipMallocAsync(&mem, end ? end : 256, st) : hipMalloc(&mem, end ? end : 256);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    r.status = PK_ENOMEM;
+    std::snprintf(r.message, sizeof(r.message), "result allocation failed: %s", hipGetErrorString(e));
+    return;
+  }
+  char* m = (char*)mem;
+  auto cp = [&](uint64_t off, const void* src, uint64_t bytes) {
+    if (bytes && e == hipSuccess) e = hipMemcpyAsync(m + off, src, bytes, hipMemcpyDeviceToDevice, st);
+  };
+  cp(o1, t.keys, t.n * w);
+  cp(o2, t.counts, t.n * 4);
+  cp(o3, kt.keys, kt.n * w);
+  cp(o4, kt.counts, kt.n * 4);
+  cp(o5, mt.nodes, mt.n * sizeof(pk_mnode));
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    r.status = PK_EHIP;
+    std::snprintf(r.message, sizeof(r.message), "result copy failed: %s", hipGetErrorString(e));
+  }
+  j->mem = mem;
+  j->mem_bytes = end;
+  r.table = pk_table{t.n, t.words, t.k, (const uint64_t*)(m + o1), (const int32_t*)(m + o2)};
+  r.kept = pk_table{kt.n, t.words, t.k, (const uint64_t*)(m + o3), (const int32_t*)(m + o4)};
+  r.nodes = pk_mn_table{mt.n, t.k, (const pk_mnode*)(m + o5)};
+}
+
+void pk_server::worker(int slot) {
+  (void)hipSetDevice(device);
+  const uint64_t slot_keep = budget / (uint64_t)(n_slots > 0 ? n_slots : 1);
+  for (;;) {
+    Job* j;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_q.wait(lk, [&] { return stop || admissible(); });
+      if (stop) return;
+      j = queue.front();
+      queue.pop_front();
+      j->state = 1;
+      running++;
+      running_est += j->res.est_bytes;
+      j->res.t_start = now();
+    }
+    run(slot, j);
+    // a slot that grew past its share gives its buffers back
+    if (pk_ctx_device_bytes(ctx[slot]) > slot_keep) (void)pk_ctx_release_buffers(ctx[slot]);
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      running--;
+      running_est -= j->res.est_bytes;
+      resident += j->mem_bytes;
+      j->res.t_end = now();
+      j->state = 2;
+    }
+    cv_done.notify_all();
+    cv_q.notify_all();
+  }
+}
+
+extern "C" {
+
+int pk_server_create(pk_server** out, int device, int n_slots, uint64_t hbm_budget_bytes) {
+  if (!out || n_slots < 1 || n_slots > 64) return PK_EINVAL;
+  *out = nullptr;
+  if (hipSetDevice(device) != hipSuccess) return PK_EHIP;
+  pk_server* s = new pk_server();
+  s->device = device;
+  s->n_slots = n_slots;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    delete s;
+    return PK_EHIP;
+  }
+  s->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(fr * 0.9);
+  int pools = 0;
+  if (hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, device) == hipSuccess)
+    s->pools = pools != 0;
+  if (hipStreamCreateWithFlags(&s->free_stream, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return PK_EHIP;
+  }
+  for (int i = 0; i < n_slots; i++) {
+    pk_ctx* c = nullptr;
+    int rc = pk_ctx_create(&c, device, 0);
+    if (rc) {
+      for (auto* x : s->ctx) pk_ctx_destroy(x);
+      (void)hipStreamDestroy(s->free_stream);
+      delete s;
+      return rc;
+    }
+    s->ctx.push_back(c);
+  }
+  for (int i = 0; i < n_slots; i++) s->workers.emplace_back([s, i] { s->worker(i); });
+  *out = s;
+  return PK_OK;
+}
+
+void pk_server_destroy(pk_server* s) {
+  if (!s) return;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (Job* j : s->queue) {
+      j->state = 2;
+      j->res.status = PK_ESTATE;
+      std::snprintf(j->res.message, sizeof(j->res.message), "server destroyed before the job ran");
+    }
+    s->queue.clear();
+    s->cv_done.wait(lk, [&] { return s->running == 0; });
+    s->stop = true;
+  }
+  s->cv_q.notify_all();
+  s->cv_done.notify_all();
+  for (auto& t : s->workers) t.join();
+  (void)hipSetDevice(s->device);
+  for (auto& kv : s->jobs) {
+    if (kv.second->mem) {
+      if (s->pools) (void)hipFreeAsync(kv.second->mem, s->free_stream);
+      else (void)hipFree(kv.second->mem);
+    }
+    delete kv.second;
+  }
+  (void)hipStreamSynchronize(s->free_stream);
+  (void)hipStreamDestroy(s->free_stream);
+  for (auto* c : s->ctx) pk_ctx_destroy(c);
+  delete s;
+}
+
+double pk_server_now(const pk_server* s) { return s ? s->now() : 0.0; }
+
+const char* pk_server_last_error(const pk_server* s) { return s ? s->err.c_str() : "null server"; }
+
+static int submit_locked(pk_server* s, const pk_job& spec, uint64_t* ticket) {
+  if (spec.k < 8 || spec.k > 64) return s->fail(PK_EINVAL, "k must be in [8, 64]");
+  if (!spec.fasta && spec.nbytes) return s->fail(PK_EINVAL, "null fasta");
+  if (spec.want_nodes && spec.t < 0) return s->fail(PK_EINVAL, "macro nodes need a cutoff (t >= 0)");
+  Job* j = new Job();
+  j->spec = spec;
+  j->ticket = s->next_ticket++;
+  j->res.est_bytes = estimate_bytes(spec);
+  j->res.t_submit = s->now();
+  s->jobs[j->ticket] = j;
+  s->queue.push_back(j);
+  *ticket = j->ticket;
+  return PK_OK;
+}
+
+int pk_server_submit(pk_server* s, const pk_job* job, uint64_t* ticket) {
+  if (!s || !job || !ticket) return PK_EINVAL;
+  int rc;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    rc = submit_locked(s, *job, ticket);
+  }
+  s->cv_q.notify_all();
+  return rc;
+}
+
+int pk_server_submit_batch(pk_server* s, const pk_job* jobs, int n, uint64_t* tickets) {
+  if (!s || n < 0 || (n && (!jobs || !tickets))) return PK_EINVAL;
+  std::vector<int> order((size_t)n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return jobs[a].nbytes > jobs[b].nbytes; });
+  int rc = PK_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (int i : order) {
+      rc = submit_locked(s, jobs[i], &tickets[i]);
+      if (rc) break;
+    }
+  }
+  s->cv_q.notify_all();
+  return rc;
+}
+
+static int finished(pk_server* s, uint64_t ticket, pk_job_result* out, bool block) {
+  if (!s || !out) return PK_EINVAL;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(ticket);
+  if (it == s->jobs.end()) return s->fail(PK_EINVAL, "unknown ticket");
+  Job* j = it->second;
+  if (j->state != 2) {
+    if (!block) return PK_EAGAIN;
+    s->cv_done.wait(lk, [&] { return j->state == 2; });
+  }
+  *out = j->res;
+  return PK_OK;
+}
+
+int pk_server_wait(pk_server* s, uint64_t ticket, pk_job_result* out) {
+  return finished(s, ticket, out, true);
+}
+
+int pk_server_poll(pk_server* s, uint64_t ticket, pk_job_result* out) {
+  return finished(s, ticket, out, false);
+}
+
+int pk_server_download(pk_server* s, uint64_t ticket, uint64_t* keys, int32_t* counts,
+                       uint64_t* kept_keys, int32_t* kept_counts, uint64_t* hist_t,
+                       pk_mnode* nodes) {
+  if (!s) return PK_EINVAL;
+  Job* j;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->jobs.find(ticket);
+    if (it == s->jobs.end()) return s->fail(PK_EINVAL, "unknown ticket");
+    j = it->second;
+    if (j->state != 2) return s->fail(PK_ESTATE, "job has not finished");
+  }
+  const pk_job_result& r = j->res;
+  if (hipSetDevice(s->device) != hipSuccess) return s->fail(PK_EHIP, "hipSetDevice failed");
+  const uint64_t w = (uint64_t)r.table.words * 8;
+  auto get = [&](void* dst, const void* src, uint64_t bytes) -> hipError_t {
+    if (!dst || !bytes || !src) return hipSuccess;
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  };
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = get(keys, r.table.keys, r.table.n * w);
+  if (e == hipSuccess) e = get(counts, r.table.counts, r.table.n * 4);
+  if (e == hipSuccess) e = get(kept_keys, r.kept.keys, r.kept.n * w);
+  if (e == hipSuccess) e = get(kept_counts, r.kept.counts, r.kept.n * 4);
+  if (e == hipSuccess) e = get(nodes, r.nodes.nodes, r.nodes.n * sizeof(pk_mnode));
+  if (e != hipSuccess) return s->fail(PK_EHIP, std::string("download failed: ") + hipGetErrorString(e));
+  if (hist_t && !j->hist.empty()) std::memcpy(hist_t, j->hist.data(), j->hist.size() * 8);
+  return PK_OK;
+}
+
+int pk_server_release(pk_server* s, uint64_t ticket) {
+  if (!s) return PK_EINVAL;
+  Job* j;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->jobs.find(ticket);
+    if (it == s->jobs.end()) return s->fail(PK_EINVAL, "unknown ticket");
+    j = it->second;
+    if (j->state != 2) return s->fail(PK_ESTATE, "job has not finished");
+    s->jobs.erase(it);
+    s->resident -= j->mem_bytes;
+  }
+  if (j->mem) {
+    (void)hipSetDevice(s->device);
+    if (s->pools) (void)hipFreeAsync(j->mem, s->free_stream);
+    else (void)hipFree(j->mem);
+  }
+  delete j;
+  s->cv_q.notify_all();
+  return PK_OK;
+}
+
+}  // extern "C"
