@@ -8,6 +8,11 @@ line (metric, value in k-mer instances per second, roofline of the dominant
 kernel, CPU baseline of the oracle on a bounded sample).
 
   python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload bacterial]
+                  [--forms plain,batched,served] [--sets 256] [--slots 4] [--load 0.8]
+
+The batched and served forms run after the plain timed steps, on the
+256-set workload (see bench_batched_served), and are reported in the same
+JSON line under "batched" and "served".
 """
 from __future__ import annotations
 
@@ -78,6 +83,132 @@ def cpu_baseline(fa, k, t, frac):
                       f"{dt:.1f} s"}
 
 
+def make_batched_sets(num_sets, threads=8):
+    """FASTA bytes of the batched workload's read sets (generated in parallel)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from pakman_plus_amd import gen
+    specs = gen.config_batched(num_sets)
+    with ThreadPoolExecutor(threads) as ex:
+        fas = list(ex.map(gen.make_fasta, specs))
+    return specs, fas
+
+
+def run_plain_job(ctx, dev, k, t):
+    """One job on a plain context, synchronous: (seconds, instances, U, V, M)."""
+    from pakman_plus_amd import api
+    t0 = time.perf_counter()
+    ctx.count(None, k, device_ptr=dev.data_ptr(), nbytes=dev.numel())
+    U = ctx.table.n
+    V = M = 0
+    try:
+        ctx.cutoff(t)
+        V = ctx.n_kept
+        M = ctx.macro_nodes()
+    except api.PakHipError as e:
+        if e.code != api.PK_EEMPTY:
+            raise
+    dt = time.perf_counter() - t0
+    return dt, int(ctx.timing()["n_instances"]), U, V, M
+
+
+def bench_batched_served(args):
+    """Batched (256 uneven read sets, k=32, one submission) and served (Poisson
+    arrivals of the same sets with k from {32, 64}) against one-at-a-time plain
+    runs of the same jobs on one context.  Inputs are resident in HBM."""
+    import numpy as np
+    import torch
+    from pakman_plus_amd import api
+    t_gen = time.perf_counter()
+    specs, fas = make_batched_sets(args.sets)
+    t_gen = time.perf_counter() - t_gen
+    devs = [torch.from_numpy(f).to("cuda:0") for f in fas]
+    torch.cuda.synchronize()
+    nbytes = sum(int(f.nbytes) for f in fas)
+    del fas
+    W = 8
+    out = {}
+    # -- plain, one job at a time (k = 32) -------------------------------------
+    # warm-up grows the buffers to the largest job (u128 keys at k = 64)
+    big = sorted(range(len(devs)), key=lambda i: -devs[i].numel())
+    ctx = api.Context(0)
+    run_plain_job(ctx, devs[big[0]], 64, args.t)
+    plain32 = [run_plain_job(ctx, d, 32, args.t) for d in devs]
+    plain_s = sum(p[0] for p in plain32)
+    N = sum(p[1] for p in plain32)
+    alg = sum(alg_bytes(p[1], p[2], p[3], p[4], int(d.numel()), W)["pipeline"] for p, d in zip(plain32, devs))
+    # -- batched ------------------------------------------------------------------
+    srv = api.Server(0, n_slots=args.slots)
+    jobs = [{"k": 32, "t": args.t, "want_nodes": True, "device_ptr": d.data_ptr(), "nbytes": d.numel()}
+            for d in devs]
+    warm = srv.submit_batch([dict(jobs[i], k=64) for i in big[: args.slots]])
+    for tk in warm:
+        srv.release(srv.wait(tk).ticket)
+    t0 = time.perf_counter()
+    tickets = srv.submit_batch(jobs)
+    res = [srv.wait(tk) for tk in tickets]
+    batched_s = time.perf_counter() - t0
+    same = all(r.n_unique == p[2] and r.n_kept == p[3] and r.n_nodes == p[4] and r.n_instances == p[1]
+               for r, p in zip(res, plain32))
+    for tk in tickets:
+        srv.release(tk)
+    out["batched"] = {
+        "workload": f"{args.sets} read sets: genome log-uniform 50 kbp-5 Mbp, coverage 20-100x, "
+                    f"reads 75/100/150/250 bp, 0.5% errors, k=32, -t {args.t}",
+        "sets": args.sets, "fasta_bytes": nbytes, "kmer_instances": N, "slots": args.slots,
+        "value": N / batched_s, "unit": "k-mers/s", "seconds": batched_s,
+        "plain_value": N / plain_s, "plain_seconds": plain_s, "speedup_vs_plain": plain_s / batched_s,
+        "hbm_frac": alg / batched_s / HBM_PEAK, "plain_hbm_frac": alg / plain_s / HBM_PEAK,
+        "results_match_plain": same, "generation_s": t_gen,
+    }
+    # -- served: Poisson arrivals, k from {32, 64} ---------------------------------
+    rng = np.random.default_rng(5003)
+    order = rng.permutation(len(devs))
+    ks = rng.choice([32, 64], size=len(devs))
+    plain64 = {int(i): run_plain_job(ctx, devs[i], 64, args.t) for i in order if ks[i] == 64}
+    service = np.array([plain64[int(i)][0] if ks[i] == 64 else plain32[int(i)][0] for i in order])
+    inst = np.array([plain64[int(i)][1] if ks[i] == 64 else plain32[int(i)][1] for i in order])
+    lam = args.load * len(order) / service.sum()   # requests per second
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, size=len(order)))
+    # plain one-at-a-time FIFO with the measured service times
+    fin = np.empty(len(order))
+    tcur = 0.0
+    for i in range(len(order)):
+        tcur = max(tcur, arrivals[i]) + service[i]
+        fin[i] = tcur
+    plain_lat = fin - arrivals
+    plain_span = fin[-1] - arrivals[0]
+    tk = [None] * len(order)
+    base = srv.now() + 0.01
+    wall0 = time.perf_counter() + 0.01
+    for i, j in enumerate(order):
+        while time.perf_counter() - wall0 < arrivals[i]:
+            pass
+        d = devs[int(j)]
+        tk[i] = srv.submit(k=int(ks[j]), t=args.t, want_nodes=True, device_ptr=d.data_ptr(), nbytes=d.numel())
+    rs = [srv.wait(x) for x in tk]
+    for x in tk:
+        srv.release(x)
+    lat = np.array([r.t_end - r.t_submit for r in rs])
+    span = max(r.t_end for r in rs) - min(r.t_submit for r in rs)
+    ctx.close()
+    srv.close()
+    out["served"] = {
+        "workload": "Poisson arrivals of the batched read sets, k drawn from {32, 64}",
+        "requests": len(order), "arrival_rate_per_s": lam,
+        "arrival_rate_rule": f"{args.load} x the plain one-at-a-time capacity (jobs / sum of service times)",
+        "slots": args.slots, "kmer_instances": int(inst.sum()),
+        "value": float(inst.sum() / span), "unit": "k-mers/s",
+        "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "plain_value": float(inst.sum() / plain_span),
+        "plain_p50_ms": float(np.percentile(plain_lat, 50) * 1e3),
+        "plain_p99_ms": float(np.percentile(plain_lat, 99) * 1e3),
+        "plain_rule": "one job at a time on one context, FIFO, measured service time per job",
+    }
+    del devs
+    torch.cuda.empty_cache()
+    return out
+
+
 def load_traffic(stage, workload):
     """HBM bytes per launch of the stage's kernel, from the rocprofv3 PMC runs
     (FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a plain copy of
@@ -113,6 +244,11 @@ def main():
     ap.add_argument("--cpu-frac", type=float, default=0.25)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--forms", default="plain,batched,served",
+                    help="plain, plus the batched/served forms on the 256-set workload")
+    ap.add_argument("--sets", type=int, default=256)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--load", type=float, default=0.8)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -197,6 +333,9 @@ def main():
     }
     del dev
     ctx.close()
+    forms = set(args.forms.split(","))
+    if forms & {"batched", "served"}:
+        line.update(bench_batched_served(args))
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(fa, k, args.t, args.cpu_frac)
     print(json.dumps(line), flush=True)
