@@ -106,6 +106,37 @@ __device__ __forceinline__ ull block_excl_scan(ull v) {
   return r;
 }
 
+// Exclusive prefix sum of v over the block and the block total, without a
+// trailing barrier: the caller must pass a barrier before the next call.
+template <int NT>
+__device__ __forceinline__ ull block_scan_total(ull v, ull* total) {
+  constexpr int NW = NT / 64;
+  __shared__ ull wsum[NW + 1];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  ull x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    ull y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    const ull s = lane < NW ? wsum[lane] : 0ull;
+    ull inc = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      ull y = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += y;
+    }
+    if (lane < NW) wsum[lane] = inc - s;
+    if (lane == NW - 1) wsum[NW] = inc;
+  }
+  __syncthreads();
+  *total = wsum[NW];
+  return x - v + wsum[w];
+}
+
 template <int NT>
 __device__ __forceinline__ ull block_sum(ull v) {
   constexpr int NW = NT / 64;
@@ -449,26 +480,110 @@ struct TileCfg {
   static constexpr int TQ = L1_NT * Q;                 // 4096 (u64) / 2048 (u128): 32 KB
 };
 
-// Level-1 histogram: digit = bits [KB - d1, KB) of each k-mer (KB = key
-// bits: 2k on the main path, the whole word for the builder encoding).
-// Block b's count of digit d goes to H[d * nblk + b] (digit-major), so an
-// exclusive scan of H gives every block the output base of each of its
-// digit runs.
+// Level-1 producer.  A block owns rpb consecutive reads and walks them in
+// chunks: a chunk is the longest run of <= L1_NT consecutive reads whose
+// bytes fit in the LDS staging buffer, loaded with coalesced 8-byte loads;
+// thread t then walks read t of the chunk byte by byte from LDS with the
+// rolling update (a read too long for the buffer is walked from global
+// memory, alone in its chunk).  Records are in file order, so a chunk's
+// bytes are one contiguous range.
+constexpr int L1_SB = 32768;  // staging bytes
+
+struct ChunkLds {
+  ull sb[L1_SB / 8];
+  ull base;   // first staged byte (8-byte aligned)
+  int m;      // reads in the chunk
+  int lds;    // bytes are staged
+};
+
+// Sets up the chunk starting at read c0 (< r1) and stages its bytes.
+__device__ __forceinline__ void stage_chunk(ChunkLds& C, const uint8_t* fa, uint64_t n, const ull* rstart,
+                                            const uint32_t* rlen, uint64_t c0, uint64_t r1, int NT) {
+  const uint64_t r = c0 + threadIdx.x;
+  const ull first = rstart[c0] & ~7ull;
+  int fits = 0;
+  if (r < r1 && (int)threadIdx.x < NT) fits = rstart[r] + rlen[r] - first <= (ull)L1_SB;
+  // records ascend in the file: the reads that fit are a prefix
+  const int m = (int)block_sum<L1_NT>((ull)fits);
+  if (threadIdx.x == 0) {
+    C.base = first;
+    C.m = m > 0 ? m : 1;
+    C.lds = m > 0;
+  }
+  __syncthreads();
+  if (C.lds) {
+    const uint64_t last = c0 + C.m - 1;
+    const ull end = rstart[last] + rlen[last];
+    const ull nw = (end - first + 7) / 8;
+    for (ull i = threadIdx.x; i < nw; i += NT) C.sb[i] = load_word8(fa, n, first + 8 * i);
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t chunk_byte(const ChunkLds& C, const uint8_t* fa, ull p) {
+  if (C.lds) return reinterpret_cast<const uint8_t*>(C.sb)[p - C.base];
+  return fa[p];
+}
+
+// Walks up to Q more k-mers of the thread's read (state s, i, km): emits
+// through emit(q, key); returns how many.
+template <class Key, int ENC, int Q, class Emit>
+__device__ __forceinline__ int walk_read(const ChunkLds& C, const uint8_t* fa, ull s, uint32_t len, uint32_t& i,
+                                         Key& km, int K, Key mask, Emit&& emit) {
+  int q = 0;
+  if (C.lds) {
+    // 8 staged bytes per LDS load
+    while (q < Q && i < len) {
+      const ull off = s + i - C.base;
+      ull w = C.sb[off >> 3] >> ((off & 7) * 8);
+      int nb = 8 - (int)(off & 7);
+      while (nb > 0 && q < Q && i < len) {
+        km = ((km << 2) | (Key)base_code<ENC>((uint32_t)w & 0xFFu)) & mask;
+        w >>= 8;
+        nb--;
+        i++;
+        if (i >= (uint32_t)K) emit(q++, km);
+      }
+    }
+    return q;
+  }
+  while (q < Q && i < len) {
+    const uint32_t c = fa[s + i];
+    km = ((km << 2) | (Key)base_code<ENC>(c)) & mask;
+    i++;
+    if (i >= (uint32_t)K) emit(q++, km);
+  }
+  return q;
+}
+
+// Level-1 histogram: digit = bits [KB - d1, KB) of each key (KB = key bits:
+// 2K on the main path, the whole word for the builder encoding).  Block b's
+// count of digit d goes to H[d * nblk + b] (digit-major), so an exclusive
+// scan of H gives every block the output base of each of its digit runs.
 template <class Key, int ENC>
 __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n, const ull* rstart,
                                                    const uint32_t* rlen, uint64_t R, int K, int KB,
                                                    Key mask, int d1, int rpb, uint32_t* H) {
+  __shared__ ChunkLds C;
   __shared__ uint32_t h[256];
   const int nb = 1 << d1;
   for (int i = threadIdx.x; i < nb; i += L1_NT) h[i] = 0;
-  __syncthreads();
   const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
   const uint64_t r1 = r0 + rpb < R ? r0 + rpb : R;
   const int bpos = KB - d1;
-  for (uint64_t r = r0 + threadIdx.x; r < r1; r += L1_NT)
-    for_each_kmer<Key, ENC>(fa, n, rstart[r], rlen[r], K, mask,
-                       [&](Key km) { atomicAdd(&h[key_digit<Key>(km, bpos, d1)], 1u); });
-  __syncthreads();
+  for (uint64_t c0 = r0; c0 < r1;) {
+    stage_chunk(C, fa, n, rstart, rlen, c0, r1, L1_NT);
+    const int m = C.m;
+    if ((int)threadIdx.x < m) {
+      const uint64_t r = c0 + threadIdx.x;
+      uint32_t i = 0;
+      Key km = 0;
+      walk_read<Key, ENC, 0x7fffffff>(C, fa, rstart[r], rlen[r], i, km, K, mask,
+                                      [&](int, Key x) { atomicAdd(&h[key_digit<Key>(x, bpos, d1)], 1u); });
+    }
+    c0 += m;
+    __syncthreads();
+  }
   for (int i = threadIdx.x; i < nb; i += L1_NT) H[(ull)i * gridDim.x + blockIdx.x] = h[i];
 }
 
@@ -515,10 +630,10 @@ __device__ __forceinline__ void tile_split_write(TileLds<Key, NT>& L, int np, Ge
   __syncthreads();
 }
 
-// Level-1 scatter fused with the parse.  Every thread walks its reads
-// (r0 + tid, r0 + tid + L1_NT, ...) with the rolling update and produces up
-// to Q k-mers per tile into registers; the block then splits the tile.  The
-// block's digit bases come from the scanned histogram O (same layout as H).
+// Level-1 scatter fused with the parse: per chunk, every thread produces up
+// to Q keys of its read per tile (into its T1 slots, then registers); the
+// block splits the tile by digit in LDS and writes contiguous runs at its
+// digit bases (scanned histogram O, same layout as H).
 template <class Key, int ENC>
 __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_t n,
                                                       const ull* rstart, const uint32_t* rlen,
@@ -526,89 +641,62 @@ __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_
                                                       int rpb, const ull* O, Key* S1) {
   constexpr int Q = TileCfg<Key>::Q;
   __shared__ TileLds<Key, L1_NT> L;
+  __shared__ ChunkLds C;
   const int nb = 1 << d1;
-  for (int i = threadIdx.x; i < nb; i += L1_NT) L.base[i] = O[(ull)i * gridDim.x + blockIdx.x];
+  for (int i = threadIdx.x; i < nb; i += L1_NT) {
+    L.base[i] = O[(ull)i * gridDim.x + blockIdx.x];
+    L.h[i] = 0;
+  }
   const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
   const uint64_t r1 = r0 + rpb < R ? r0 + rpb : R;
   const int bpos = KB - d1;
-  const bool aligned = (reinterpret_cast<uintptr_t>(fa) & 7) == 0;
-  uint64_t r = r0 + threadIdx.x;
-  uint64_t s = 0;
-  uint32_t len = 0, i = 0;
-  if (r < r1) {
-    s = rstart[r];
-    len = rlen[r];
-  }
-  Key km = 0;
-  uint64_t wa = ~0ull, w = 0;
-  while (__syncthreads_or(r < r1)) {
-    Key kr[Q];
-    int np = 0;
+  for (uint64_t c0 = r0; c0 < r1;) {
+    stage_chunk(C, fa, n, rstart, rlen, c0, r1, L1_NT);
+    const int m = C.m;
+    const bool mine = (int)threadIdx.x < m;
+    const ull s = mine ? rstart[c0 + threadIdx.x] : 0;
+    const uint32_t len = mine ? rlen[c0 + threadIdx.x] : 0;
+    uint32_t i = 0;
+    Key km = 0;
+    // Barriers per tile: the loop test, histogram done, two in the scan,
+    // ranks ready, tile sorted.  Each thread resets its own histogram entry
+    // after reading it; the loop test orders that before the next tile's
+    // atomics, and the write-out's T1 reads before the next walk's stores.
+    while (__syncthreads_or(i < len)) {
+      Key* stage = L.T1 + threadIdx.x * Q;
+      const int np = walk_read<Key, ENC, Q>(C, fa, s, len, i, km, K, mask,
+                                            [&](int q, Key x) { stage[q] = x; });
+      Key kr[Q];
 #pragma unroll
-    for (int j = 0; j < Q; j++) {
-      kr[j] = Key(0);
-      bool got = false;
-      while (!got && r < r1) {
-        if (i >= len) {
-          r += L1_NT;
-          i = 0;
-          km = 0;
-          if (r < r1) {
-            s = rstart[r];
-            len = rlen[r];
-          }
-          continue;
-        }
-        const uint64_t p = s + i;
-        uint32_t c;
-        if (aligned) {
-          const uint64_t a = p & ~7ull;
-          if (a != wa) {
-            wa = a;
-            w = load_word8(fa, n, a);
-          }
-          c = (uint32_t)(w >> ((p & 7) * 8)) & 0xFFu;
-        } else {
-          c = fa[p];
-        }
-        km = ((km << 2) | (Key)base_code<ENC>(c)) & mask;
-        i++;
-        if (i >= (uint32_t)K) {
-          kr[j] = km;
-          got = true;
+      for (int q = 0; q < Q; q++) kr[q] = q < np ? stage[q] : Key(0);
+#pragma unroll
+      for (int q = 0; q < Q; q++)
+        if (q < np) atomicAdd(&L.h[key_digit<Key>(kr[q], bpos, d1)], 1u);
+      __syncthreads();
+      const uint32_t v = (int)threadIdx.x < nb ? L.h[threadIdx.x] : 0u;
+      ull tot;
+      const ull ex = block_scan_total<L1_NT>(v, &tot);
+      if ((int)threadIdx.x < nb) {
+        L.tcur[threadIdx.x] = (uint32_t)ex;
+        L.delta[threadIdx.x] = (long long)L.base[threadIdx.x] - (long long)ex;
+        L.base[threadIdx.x] += v;
+        L.h[threadIdx.x] = 0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        if (q < np) {
+          const uint32_t d = key_digit<Key>(kr[q], bpos, d1);
+          const uint32_t pos = atomicAdd(&L.tcur[d], 1u);
+          L.T1[pos] = kr[q];
+          L.D1[pos] = (uint8_t)d;
         }
       }
-      np += got ? 1 : 0;
+      __syncthreads();
+      const int total = (int)tot;
+      for (int p = threadIdx.x; p < total; p += L1_NT) S1[L.delta[L.D1[p]] + p] = L.T1[p];
     }
-    // produced keys are kr[0..np) (a thread stops producing only at the end
-    // of its reads, so the filled slots are a prefix)
-    const int nbits = d1;
-    for (int q = threadIdx.x; q < (1 << nbits); q += L1_NT) L.h[q] = 0;
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < Q; j++)
-      if (j < np) atomicAdd(&L.h[key_digit<Key>(kr[j], bpos, nbits)], 1u);
-    __syncthreads();
-    const uint32_t v = (int)threadIdx.x < nb ? L.h[threadIdx.x] : 0u;
-    const ull ex = block_excl_scan<L1_NT>(v);
-    if ((int)threadIdx.x < nb) {
-      L.tcur[threadIdx.x] = (uint32_t)ex;
-      L.delta[threadIdx.x] = (long long)L.base[threadIdx.x] - (long long)ex;
-    }
-    const int total = (int)block_sum<L1_NT>(v);
-#pragma unroll
-    for (int j = 0; j < Q; j++) {
-      if (j < np) {
-        const uint32_t d = key_digit<Key>(kr[j], bpos, nbits);
-        const uint32_t pos = atomicAdd(&L.tcur[d], 1u);
-        L.T1[pos] = kr[j];
-        L.D1[pos] = (uint8_t)d;
-      }
-    }
-    __syncthreads();
-    for (int p = threadIdx.x; p < total; p += L1_NT) S1[L.delta[L.D1[p]] + p] = L.T1[p];
-    if ((int)threadIdx.x < nb) L.base[threadIdx.x] += v;
-    __syncthreads();
+    c0 += m;
   }
 }
 
@@ -1986,7 +2074,8 @@ int records_builder(pk_ctx* c, const uint8_t* fa, uint64_t n, int klen, uint64_t
   return PK_OK;
 }
 
-// Count the k-mers of the read table (rstart, rlen)[0..R), N instances.
+// Count the k-mers of the read table (rstart, rlen)[0..R) (records in file
+// order), N instances.
 // K = window length, KB = key bits (2K on the main path, the whole word for
 // the builder encoding), mask = word mask of the rolling update.
 template <class Key, int ENC>
@@ -2017,7 +2106,9 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   const int btot = choose_bits(N, KB / 2, CAP);
   const int d1 = btot < 8 ? btot : 8;
   const uint64_t B1 = 1ull << d1;
-  const int rpb = 2048;  // reads per block
+  // reads per block: up to 8 chunks of 256, fewer for small inputs so that
+  // the grid still spreads over the CUs
+  const int rpb = L1_NT * (int)std::min<uint64_t>(8, std::max<uint64_t>(1, R / (L1_NT * 1024ull)));
   const uint64_t nblk = (R + rpb - 1) / rpb;
   ull *boff, *O;
   uint32_t* H;
