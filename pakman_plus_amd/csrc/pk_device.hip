@@ -527,9 +527,9 @@ __device__ __forceinline__ uint32_t chunk_byte(const ChunkLds& C, const uint8_t*
 
 // Walks up to Q more k-mers of the thread's read (state s, i, km): emits
 // through emit(q, key); returns how many.
-template <class Key, int ENC, int Q, class Emit>
+template <class Key, int ENC, class Emit>
 __device__ __forceinline__ int walk_read(const ChunkLds& C, const uint8_t* fa, ull s, uint32_t len, uint32_t& i,
-                                         Key& km, int K, Key mask, Emit&& emit) {
+                                         Key& km, int K, Key mask, int Q, Emit&& emit) {
   int q = 0;
   if (C.lds) {
     // 8 staged bytes per LDS load
@@ -578,8 +578,8 @@ __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n
       const uint64_t r = c0 + threadIdx.x;
       uint32_t i = 0;
       Key km = 0;
-      walk_read<Key, ENC, 0x7fffffff>(C, fa, rstart[r], rlen[r], i, km, K, mask,
-                                      [&](int, Key x) { atomicAdd(&h[key_digit<Key>(x, bpos, d1)], 1u); });
+      walk_read<Key, ENC>(C, fa, rstart[r], rlen[r], i, km, K, mask, 0x7fffffff,
+                          [&](int, Key x) { atomicAdd(&h[key_digit<Key>(x, bpos, d1)], 1u); });
     }
     c0 += m;
     __syncthreads();
@@ -634,11 +634,13 @@ __device__ __forceinline__ void tile_split_write(TileLds<Key, NT>& L, int np, Ge
 // to Q keys of its read per tile (into its T1 slots, then registers); the
 // block splits the tile by digit in LDS and writes contiguous runs at its
 // digit bases (scanned histogram O, same layout as H).
+// qt (<= Q) keys per thread per tile: chosen so that a typical read's
+// k-mers fill whole tiles (69 k-mers: 5 tiles of 14 rather than 16+16+16+16+5).
 template <class Key, int ENC>
 __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_t n,
                                                       const ull* rstart, const uint32_t* rlen,
                                                       uint64_t R, int K, int KB, Key mask, int d1,
-                                                      int rpb, const ull* O, Key* S1) {
+                                                      int rpb, int qt, const ull* O, Key* S1) {
   constexpr int Q = TileCfg<Key>::Q;
   __shared__ TileLds<Key, L1_NT> L;
   __shared__ ChunkLds C;
@@ -664,8 +666,8 @@ __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_
     // atomics, and the write-out's T1 reads before the next walk's stores.
     while (__syncthreads_or(i < len)) {
       Key* stage = L.T1 + threadIdx.x * Q;
-      const int np = walk_read<Key, ENC, Q>(C, fa, s, len, i, km, K, mask,
-                                            [&](int q, Key x) { stage[q] = x; });
+      const int np = walk_read<Key, ENC>(C, fa, s, len, i, km, K, mask, qt,
+                                         [&](int q, Key x) { stage[q] = x; });
       Key kr[Q];
 #pragma unroll
       for (int q = 0; q < Q; q++) kr[q] = q < np ? stage[q] : Key(0);
@@ -855,6 +857,10 @@ struct HashCfg<u128> {
 };
 // tasks with more instances than this are split without trying
 constexpr uint64_t TASK_MAX_INSTANCES = 1u << 18;
+// linear-probe bound of an insert: a table too full to place a key within
+// it sends the task to the next pass split (the host sizes buckets for a
+// load of about 0.75, where probes average a few slots)
+constexpr int HASH_PROBES = 256;
 
 __device__ __forceinline__ ull mix64(ull x) {
   x ^= x >> 33;
@@ -881,13 +887,12 @@ struct HashTable<uint64_t> {
       hc[i] = 0;
     }
   }
-  // returns false when the table is full
-  __device__ __forceinline__ bool insert(uint64_t k, uint32_t* ndist, int limit, int* over) {
+  // returns false when no free slot is found within HASH_PROBES probes
+  __device__ __forceinline__ bool insert(uint64_t k) {
     uint32_t slot = (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> 40) & (T - 1);
-    for (int probe = 0; probe < T; probe++) {
+    for (int probe = 0; probe < HASH_PROBES; probe++) {
       const ull prev = atomicCAS(&hk[slot], empty, (ull)k);
       if (prev == empty) {
-        if ((int)atomicAdd(ndist, 1u) >= limit) *over = 1;
         atomicAdd(&hc[slot], 1u);
         return true;
       }
@@ -924,18 +929,14 @@ struct HashTable<u128> {
     }
   }
   __device__ __forceinline__ uint32_t home(ull f) const { return (uint32_t)(f >> 40) & (T - 1); }
-  __device__ __forceinline__ bool insert(u128 k, uint32_t* ndist, int limit, int* over) {
+  __device__ __forceinline__ bool insert(u128 k) {
     const ull f = fp(k);
-    if (f == EMPTY) {
-      *over = 1;
-      return false;
-    }
+    if (f == EMPTY) return false;
     uint32_t slot = home(f);
-    for (int probe = 0; probe < T; probe++) {
+    for (int probe = 0; probe < HASH_PROBES; probe++) {
       const ull prev = atomicCAS(&hf[slot], EMPTY, f);
       if (prev == EMPTY) {
         hkf[slot] = k;
-        if ((int)atomicAdd(ndist, 1u) >= limit) *over = 1;
         atomicAdd(&hc[slot], 1u);
         return true;
       }
@@ -1008,13 +1009,11 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
                                                        uint64_t ntasks, int K, int fixed,
                                                        long long* tu, ull* ovf_count, ull* ovf_list) {
   constexpr int T = HashCfg<Key>::T;
-  constexpr int LIMIT = HashCfg<Key>::LIMIT;
   constexpr int PER = T / BS_NT;  // table slots per thread
   constexpr int HB = sizeof(Key) == 8 ? 16 : 8;
   __shared__ HashLds<Key> lds;
   __shared__ uint32_t hs[SUB_BINS];
   __shared__ uint32_t he[SUB_BINS];
-  __shared__ uint32_t ndist;
   __shared__ int over;
 
   const uint64_t task = blockIdx.x;
@@ -1042,10 +1041,7 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
     bool failed = false;
     for (int pass = 0; pass < (1 << s) && !failed; pass++) {
       ht.init(threadIdx.x, BS_NT);
-      if (threadIdx.x == 0) {
-        ndist = 0;
-        over = 0;
-      }
+      if (threadIdx.x == 0) over = 0;
       __syncthreads();
       // keys are loaded HB at a time into registers so that the global-load
       // latency is paid once per batch, not once per insert
@@ -1060,8 +1056,7 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
 #pragma unroll
         for (int j = 0; j < HB; j++) {
           const uint64_t idx = b + threadIdx.x + (uint64_t)j * BS_NT;
-          if (idx < n && (int)key_digit<Key>(kb[j], rem - s, s) == pass &&
-              !ht.insert(kb[j], &ndist, LIMIT, &over))
+          if (idx < n && (int)key_digit<Key>(kb[j], rem - s, s) == pass && !ht.insert(kb[j]))
             over = 1;
         }
       }
@@ -1101,6 +1096,7 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
       const int nbins = 1 << dbits;
       Key rk[PER];
       uint32_t rc[PER];
+      ull ndk;  // distinct keys of the pass
       for (int i = threadIdx.x; i < nbins; i += BS_NT) hs[i] = 0;
       __syncthreads();
 #pragma unroll
@@ -1121,7 +1117,7 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
           v[i] = bb < nbins ? hs[bb] : 0u;
           sm += v[i];
         }
-        ull ex = block_excl_scan<BS_NT>(sm);
+        ull ex = block_scan_total<BS_NT>(sm, &ndk);
 #pragma unroll
         for (int i = 0; i < BPT; i++) {
           const int bb = threadIdx.x * BPT + i;
@@ -1144,7 +1140,7 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
         }
       }
       __syncthreads();
-      const uint32_t nd = ndist;
+      const uint32_t nd = (uint32_t)ndk;
       for (uint32_t i = threadIdx.x; i < nd; i += BS_NT) {
         const Key v = ok[i];
         const int d = (int)key_digit<Key>(v, dpos, dbits);
@@ -2110,6 +2106,13 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   // the grid still spreads over the CUs
   const int rpb = L1_NT * (int)std::min<uint64_t>(8, std::max<uint64_t>(1, R / (L1_NT * 1024ull)));
   const uint64_t nblk = (R + rpb - 1) / rpb;
+  int qt = TileCfg<Key>::Q;
+  {
+    const uint64_t kpr = (N + R - 1) / R;  // k-mers per read (mean)
+    const uint64_t tiles = (kpr + qt - 1) / qt;
+    if (tiles > 0) qt = (int)((kpr + tiles - 1) / tiles);
+    if (qt < 1) qt = 1;
+  }
   ull *boff, *O;
   uint32_t* H;
   Key* S2;
@@ -2131,7 +2134,7 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[4], st));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key, ENC>), dim3((unsigned)nblk), dim3(L1_NT), 0, st,
-                     fa, n, rstart, rlen, R, K, KB, mask, d1, rpb, O, S);
+                     fa, n, rstart, rlen, R, K, KB, mask, d1, rpb, qt, O, S);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[16], st));
   uint64_t B = B1;
