@@ -472,7 +472,7 @@ __device__ __forceinline__ void for_each_kmer(const uint8_t* fa, uint64_t n, uin
 //   level 2+ k_ms_hist / k_ms_scatter: from the previous level's array
 
 constexpr int L1_NT = 256;
-constexpr int MS_NT = 256;
+constexpr int MS_NT = 512;
 
 template <class Key>
 struct TileCfg {
@@ -593,8 +593,8 @@ __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n
 // out[base[d] - toff[d] + p].  Keys come from get(j), j < np.
 template <class Key, int NT>
 struct TileLds {
-  Key T1[TileCfg<Key>::TQ];
-  uint8_t D1[TileCfg<Key>::TQ];
+  Key T1[NT * TileCfg<Key>::Q];
+  uint8_t D1[NT * TileCfg<Key>::Q];
   uint32_t h[256];
   uint32_t tcur[256];
   ull base[256];
@@ -748,7 +748,7 @@ __device__ __forceinline__ ull ms_index(const ull* tp, int a, int s, ull t, int 
 template <class Key>
 __global__ __launch_bounds__(MS_NT) void k_ms_hist(const Key* Sin, const ull* boff, const ull* tp, int nb,
                                                    int dpos, int dbits, uint32_t* H) {
-  constexpr int TQ = TileCfg<Key>::TQ;
+  constexpr int TQ = MS_NT * TileCfg<Key>::Q;
   __shared__ uint32_t h[256];
   const ull blk = blockIdx.x;
   if (blk >= tp[nb]) return;
@@ -781,7 +781,7 @@ __global__ __launch_bounds__(MS_NT) void k_ms_scatter(const Key* Sin, const ull*
                                                       int nb, int dpos, int dbits, const ull* O,
                                                       Key* Sout) {
   constexpr int Q = TileCfg<Key>::Q;
-  constexpr int TQ = TileCfg<Key>::TQ;
+  constexpr int TQ = MS_NT * TileCfg<Key>::Q;
   __shared__ TileLds<Key, MS_NT> L;
   const ull blk = blockIdx.x;
   if (blk >= tp[nb]) return;
@@ -1385,7 +1385,7 @@ __global__ __launch_bounds__(CUT_NT) void k_kept_write(const Key* TK, const int3
 
 constexpr int MN_NT = 512;
 constexpr int MN_PC = 1024;    // pred entries per range
-constexpr int MN_ECAP = 4096;  // LDS capacity (entries of all five lists)
+constexpr int MN_ECAP = 3072;  // LDS capacity (entries of all five lists)
 
 struct MnPlan {
   ull lo[5];
@@ -1535,17 +1535,31 @@ struct MnCursor {
   }
 };
 
+// LDS ranges (GLOBAL == false) use the launch's dynamic LDS: the range's
+// five lists (node keys) | per rank: start bit 31, list << 16 | index |
+// per node: rank of its first element.
+template <class Key>
+constexpr size_t mn_lds_bytes() {
+  return MN_ECAP * (sizeof(Key) + 2 * sizeof(uint32_t));
+}
+
 // One block per range.  GLOBAL == false: ranges with e <= MN_ECAP, lists in
 // LDS.  GLOBAL == true: the remaining ranges, lists and rank flags in global
 // memory (gflags, indexed from plan.base).  Each thread handles a contiguous
-// chunk of the concatenated lists and walks its positions in the other lists.
+// chunk of the concatenated lists and walks its positions in the other lists
+// (pass 1: rank of every element in the merged order and whether it starts
+// a node); pass 2 numbers the nodes; pass 3 writes the records.
 template <class Key, bool GLOBAL>
 __global__ __launch_bounds__(MN_NT) void k_mn_build(const Key* KK, const int32_t* KC, int K,
                                                     const MnPlan* plan, uint64_t G, MnRec* NT,
                                                     ull* ng, uint32_t* gflags) {
-  __shared__ Key lk[GLOBAL ? 1 : MN_ECAP];
-  __shared__ uint32_t lflag[GLOBAL ? 1 : MN_ECAP];
+  extern __shared__ __align__(16) unsigned char mn_smem[];
+  Key* lk = reinterpret_cast<Key*>(mn_smem);
+  uint32_t* lflag = reinterpret_cast<uint32_t*>(mn_smem + MN_ECAP * sizeof(Key));
+  uint32_t* lstart = lflag + MN_ECAP;
   __shared__ ull chunk_base;
+  __shared__ ull s_lo[5];   // list starts in the kept table (block-uniform,
+  __shared__ ull s_cum[6];  // indexed at run time from LDS, not registers)
   const uint64_t g = blockIdx.x;
   if (g >= G) return;
   const MnPlan p = plan[g];
@@ -1568,6 +1582,14 @@ __global__ __launch_bounds__(MN_NT) void k_mn_build(const Key* KK, const int32_t
     }
     cum[5] = o;
   }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+      s_lo[l] = p.lo[l];
+      s_cum[l] = cum[l];
+    }
+    s_cum[5] = cum[5];
+  }
   if (!GLOBAL) {
 #pragma unroll
     for (int l = 0; l < 5; l++) {
@@ -1576,8 +1598,8 @@ __global__ __launch_bounds__(MN_NT) void k_mn_build(const Key* KK, const int32_t
         lk[cum[l] + i] = l == 0 ? mn_pred(x) : (x & L.msk);
       }
     }
-    __syncthreads();
   }
+  __syncthreads();
   uint32_t* flags = GLOBAL ? gflags + p.base : lflag;
   const ull chunk = (E + MN_NT - 1) / MN_NT;
   const ull e0 = (ull)threadIdx.x * chunk;
@@ -1604,11 +1626,14 @@ __global__ __launch_bounds__(MN_NT) void k_mn_build(const Key* KK, const int32_t
           rank += cur.lb[o];
         }
       }
-      flags[rank] = start ? 1u : 0u;
+      if (GLOBAL) flags[rank] = start ? 1u : 0u;
+      else flags[rank] = (start ? 0x80000000u : 0u) | ((uint32_t)l << 16) | (uint32_t)j;
     }
   }
   __syncthreads();
-  // pass 2: exclusive scan of flags in rank order (chunks of MN_NT*4)
+  // pass 2: exclusive scan of the start flags in rank order (chunks of
+  // MN_NT*4).  Global ranges keep "node index + 1" per start rank; LDS
+  // ranges keep their per-rank element ids and note each node's first rank.
   if (threadIdx.x == 0) chunk_base = 0;
   __syncthreads();
   for (ull c0 = 0; c0 < E; c0 += (ull)MN_NT * 4) {
@@ -1617,15 +1642,18 @@ __global__ __launch_bounds__(MN_NT) void k_mn_build(const Key* KK, const int32_t
     ull sm = 0;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      v[i] = i0 + i < E ? flags[i0 + i] : 0u;
+      v[i] = i0 + i < E ? (GLOBAL ? flags[i0 + i] : flags[i0 + i] >> 31) : 0u;
       sm += v[i];
     }
     ull ex = block_excl_scan<MN_NT>(sm) + chunk_base;
     const ull tot = block_sum<MN_NT>(sm);
-    // store "node index + 1" for starts, 0 otherwise
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      if (i0 + i < E) flags[i0 + i] = v[i] ? (uint32_t)(ex + 1) : 0u;
+      if (GLOBAL) {
+        if (i0 + i < E) flags[i0 + i] = v[i] ? (uint32_t)(ex + 1) : 0u;
+      } else if (v[i]) {
+        lstart[ex] = (uint32_t)(i0 + i);
+      }
       ex += v[i];
     }
     __syncthreads();
@@ -1633,43 +1661,94 @@ __global__ __launch_bounds__(MN_NT) void k_mn_build(const Key* KK, const int32_t
     __syncthreads();
   }
   const ull nodes = chunk_base;
-  // pass 3: every node start assembles its record
-#pragma unroll
-  for (int l = 0; l < 5; l++) {
-    const ull a = e0 > cum[l] ? e0 : cum[l];
-    const ull b = e1 < cum[l + 1] ? e1 : cum[l + 1];
-    if (a >= b) continue;
-    MnCursor<Key, GLOBAL> cur;
-    for (ull e = a; e < b; e++) {
-      const ull j = e - cum[l];
-      const Key x = L.at(L.off[l], l == 0, j);
-      if (e == a) cur.seek(L, x); else cur.advance(L, x);
-      if (cur.lb[l] != j) continue;
-      ull rank = j;
-      bool start = true;
-#pragma unroll
-      for (int o = 0; o < 5; o++) {
-        if (o < l) {
-          rank += cur.ub[o];
-          if (cur.ub[o] != cur.lb[o]) start = false;
-        } else if (o > l) {
-          rank += cur.lb[o];
+  if (!GLOBAL) {
+    // pass 3 (LDS ranges): one thread per node, in node order, so that
+    // consecutive lanes write consecutive records.  A node's elements are
+    // consecutive in rank order: its first, then the ranks up to the next
+    // start.  Suffix b of the node: pred-list element with last base b;
+    // prefix a: the element of quarter list a + 1.
+    for (ull nd = threadIdx.x; nd < nodes; nd += MN_NT) {
+      const uint32_t r0 = lstart[nd];
+      const uint32_t id0 = lflag[r0];
+      MnRec rec;
+      set_key<Key>(rec, lk[s_cum[(id0 >> 16) & 7u] + (id0 & 0xFFFFu)]);
+      int32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0, q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+      for (ull r = r0; r < E; r++) {
+        const uint32_t id = lflag[r];
+        if (r != r0 && (id >> 31)) break;
+        const uint32_t l = (id >> 16) & 7u;
+        const ull gi = s_lo[l] + (id & 0xFFFFu);
+        const int32_t cnt = KC[gi];
+        if (l == 0) {
+          const uint32_t bse = (uint32_t)(KK[gi] & Key(3));
+          s0 = bse == 0 ? cnt : s0;
+          s1 = bse == 1 ? cnt : s1;
+          s2 = bse == 2 ? cnt : s2;
+          s3 = bse == 3 ? cnt : s3;
+        } else {
+          q0 = l == 1 ? cnt : q0;
+          q1 = l == 2 ? cnt : q1;
+          q2 = l == 3 ? cnt : q2;
+          q3 = l == 4 ? cnt : q3;
         }
       }
-      if (!start) continue;
-      const uint32_t idx1 = flags[rank];
-      MnRec rec;
-      set_key<Key>(rec, x);
+      rec.suf[0] = s0;
+      rec.suf[1] = s1;
+      rec.suf[2] = s2;
+      rec.suf[3] = s3;
+      rec.pre[0] = q0;
+      rec.pre[1] = q1;
+      rec.pre[2] = q2;
+      rec.pre[3] = q3;
+      NT[p.base + nd] = rec;
+    }
+  } else {
+    // pass 3 (global ranges): every node start assembles its record
 #pragma unroll
-      for (int q = 0; q < 4; q++) rec.pre[q] = rec.suf[q] = 0;
-      for (ull i = cur.lb[0]; i < cur.ub[0]; i++) {
-        const ull gi = p.lo[0] + i;
-        rec.suf[(int)(KK[gi] & Key(3))] = KC[gi];
+    for (int l = 0; l < 5; l++) {
+      const ull a = e0 > cum[l] ? e0 : cum[l];
+      const ull b = e1 < cum[l + 1] ? e1 : cum[l + 1];
+      if (a >= b) continue;
+      MnCursor<Key, GLOBAL> cur;
+      for (ull e = a; e < b; e++) {
+        const ull j = e - cum[l];
+        const Key x = L.at(L.off[l], l == 0, j);
+        if (e == a) cur.seek(L, x); else cur.advance(L, x);
+        if (cur.lb[l] != j) continue;
+        ull rank = j;
+        bool start = true;
+#pragma unroll
+        for (int o = 0; o < 5; o++) {
+          if (o < l) {
+            rank += cur.ub[o];
+            if (cur.ub[o] != cur.lb[o]) start = false;
+          } else if (o > l) {
+            rank += cur.lb[o];
+          }
+        }
+        if (!start) continue;
+        const uint32_t idx1 = flags[rank];
+        MnRec rec;
+        set_key<Key>(rec, x);
+        int32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        for (ull i = cur.lb[0]; i < cur.ub[0]; i++) {
+          const ull gi = p.lo[0] + i;
+          const uint32_t bse = (uint32_t)(KK[gi] & Key(3));
+          const int32_t cnt = KC[gi];
+          s0 = bse == 0 ? cnt : s0;
+          s1 = bse == 1 ? cnt : s1;
+          s2 = bse == 2 ? cnt : s2;
+          s3 = bse == 3 ? cnt : s3;
+        }
+        rec.suf[0] = s0;
+        rec.suf[1] = s1;
+        rec.suf[2] = s2;
+        rec.suf[3] = s3;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          rec.pre[q] = cur.ub[q + 1] > cur.lb[q + 1] ? KC[p.lo[q + 1] + cur.lb[q + 1]] : 0;
+        NT[p.base + idx1 - 1] = rec;
       }
-#pragma unroll
-      for (int q = 0; q < 4; q++)
-        if (cur.ub[q + 1] > cur.lb[q + 1]) rec.pre[q] = KC[p.lo[q + 1] + cur.lb[q + 1]];
-      NT[p.base + idx1 - 1] = rec;
     }
   }
   if (threadIdx.x == 0) ng[g] = nodes;
@@ -2142,7 +2221,7 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   int set = 0;
   Key* Sin = S;
   Key* Sout = S2;
-  constexpr int TQ = TileCfg<Key>::TQ;
+  constexpr int TQ = MS_NT * TileCfg<Key>::Q;
   while (fixed < btot) {
     const int db = btot - fixed < 8 ? btot - fixed : 8;
     const uint64_t B2 = B << db;
@@ -2408,8 +2487,8 @@ int mn_impl(pk_ctx* c) {
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mn_plan<Key>), dim3(grid_for(G, 256)), dim3(256), 0, st, KK,
                      V, K, q, G, plan, maxe);
   PK_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mn_build<Key, false>), dim3((unsigned)G), dim3(MN_NT), 0, st,
-                     KK, KC, K, plan, G, NT, ng, gflags);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mn_build<Key, false>), dim3((unsigned)G), dim3(MN_NT),
+                     mn_lds_bytes<Key>(), st, KK, KC, K, plan, G, NT, ng, gflags);
   PK_HIP(c, hipGetLastError());
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mn_build<Key, true>), dim3((unsigned)G), dim3(MN_NT), 0, st,
                      KK, KC, K, plan, G, NT, ng, gflags);
