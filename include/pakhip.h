@@ -113,6 +113,22 @@ int pk_ctx_release_buffers(pk_ctx* ctx);
 int pk_count(pk_ctx* ctx, const char* fasta, uint64_t nbytes, int k, int fasta_on_device,
              pk_table* out);
 
+/* One rank's share of the counted table at nranks ranks: the k-mers whose
+ * owner is `rank`, i.e. the kmer_proc_buf that rank holds after the
+ * reference's exchange (perform_kmer_counting at P = nranks,
+ * src/kmer_counting.cpp:2950-3050,3351-3458).  Owner = hash31(m) % nranks
+ * (src/distribute_kmers.cpp:95-105) of the k-mer's leftmost minimum-frequency
+ * 8-mer m, frequencies taken over the whole input (Sliding_window_l +
+ * MPI_Allreduce, src/kmer_counting.cpp:391-438,3366).  nranks == 1 is
+ * pk_count. */
+int pk_count_rank_share(pk_ctx* ctx, const char* fasta, uint64_t nbytes, int k, int nranks, int rank,
+                        int fasta_on_device, pk_table* out);
+
+/* The 8-mer frequency table of the input (4^8 uint64 entries, host array),
+ * as Sliding_window_l computes it (src/kmer_counting.cpp:391-438). */
+int pk_lmer_frequency(pk_ctx* ctx, const char* fasta, uint64_t nbytes, int fasta_on_device,
+                      uint64_t* freq_host);
+
 /* Builder-API counting (Concurrent_Kmer_Partial_Counter<kmer_t>::
  * default_count_and_process, src/concurrent_kmer_counter.hpp:2018-2082, with
  * the ctor arguments klen, read_len, delimiter ('\n') and skip_symbol ('>')

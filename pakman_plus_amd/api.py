@@ -70,7 +70,8 @@ MNODE_DTYPE = np.dtype([("key_lo", "<u8"), ("key_hi", "<u8"), ("pre_cnt", "<i4",
 
 # every symbol declared in include/pakhip.h
 EXPORTS = ("pk_ctx_create", "pk_ctx_destroy", "pk_last_error", "pk_ctx_stream",
-           "pk_ctx_device_bytes", "pk_ctx_release_buffers", "pk_count", "pk_builder_count",
+           "pk_ctx_device_bytes", "pk_ctx_release_buffers", "pk_count", "pk_count_rank_share",
+           "pk_lmer_frequency", "pk_builder_count",
            "pk_cutoff", "pk_kept_table", "pk_macro_nodes", "pk_download_table", "pk_download_kept",
            "pk_download_nodes", "pk_last_timing", "pk_server_create", "pk_server_destroy",
            "pk_server_now", "pk_server_submit", "pk_server_submit_batch", "pk_server_wait",
@@ -107,6 +108,10 @@ def load_library():
     L.pk_ctx_device_bytes.restype = u64
     L.pk_count.argtypes = [vp, vp, u64, i32, i32, ctypes.POINTER(_Table)]
     L.pk_count.restype = i32
+    L.pk_count_rank_share.argtypes = [vp, vp, u64, i32, i32, i32, i32, ctypes.POINTER(_Table)]
+    L.pk_count_rank_share.restype = i32
+    L.pk_lmer_frequency.argtypes = [vp, vp, u64, i32, vp]
+    L.pk_lmer_frequency.restype = i32
     L.pk_builder_count.argtypes = [vp, vp, u64, i32, u64, i32, ctypes.c_char, ctypes.c_char, i32,
                                    ctypes.POINTER(_Table), ctypes.POINTER(u64)]
     L.pk_builder_count.restype = i32
@@ -224,6 +229,30 @@ class Context:
         self.k = k
         self.table = TableView(int(t.n), int(t.words), int(t.k), t.keys or 0, t.counts or 0)
         return self.table
+
+    def count_rank_share(self, fasta, k: int, nranks: int, rank: int, device_ptr: int | None = None,
+                         nbytes: int | None = None) -> TableView:
+        """Rank `rank`'s share at `nranks` ranks (owner = hash31(min-frequency
+        8-mer) % nranks, kmer_counting.cpp:2950-3050)."""
+        t = _Table()
+        if device_ptr is not None:
+            rc = self._lib.pk_count_rank_share(self._h, ctypes.c_void_p(device_ptr), nbytes, k, nranks, rank,
+                                               1, ctypes.byref(t))
+        else:
+            a = _as_u8(fasta)
+            rc = self._lib.pk_count_rank_share(self._h, a.ctypes.data, a.nbytes, k, nranks, rank, 0,
+                                               ctypes.byref(t))
+        self._check(rc)
+        self.k = k
+        self.table = TableView(int(t.n), int(t.words), int(t.k), t.keys or 0, t.counts or 0)
+        return self.table
+
+    def lmer_frequency(self, fasta) -> np.ndarray:
+        """4^8 L-mer frequencies (Sliding_window_l, kmer_counting.cpp:391-438)."""
+        a = _as_u8(fasta)
+        out = np.zeros(1 << 16, dtype=np.uint64)
+        self._check(self._lib.pk_lmer_frequency(self._h, a.ctypes.data, a.nbytes, 0, out.ctypes.data))
+        return out
 
     # -- Concurrent_Kmer_Partial_Counter::default_count_and_process ------------
     def builder_count(self, fasta, klen: int, read_len: int = 100, kmer_bits: int = 64,

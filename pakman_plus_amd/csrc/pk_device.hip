@@ -527,9 +527,81 @@ __device__ __forceinline__ uint32_t chunk_byte(const ChunkLds& C, const uint8_t*
 
 // Walks up to Q more k-mers of the thread's read (state s, i, km): emits
 // through emit(q, key); returns how many.
-template <class Key, int ENC, class Emit>
+// Rank-share routing of the reference's Sliding_window (kmer_counting.cpp:
+// 2950-3050): a k-mer belongs to rank hash31(m) % P (retrieve_proc_id,
+// distribute_kmers.cpp:95-105; hash31, distribute_kmers.h:856-873) of its
+// leftmost minimum-frequency L-mer m (L = 8, frequencies of the whole input).
+// The minimum is tracked as the window slides: a new L-mer replaces it only
+// when strictly rarer, and when it leaves the window every L-mer of the
+// current k-mer is re-read from the packed k-mer (recalculate_min_lmer,
+// kmer_counting.cpp:355-388).  The rolling L-mer uses the 3-bit code like
+// the k-mer; the re-read extracts 2-bit fields of the k-mer.
+constexpr int LMER_L = 8;
+constexpr uint32_t LMER_BINS = 1u << (2 * LMER_L);
+
+struct OwnCfg {
+  const ull* freq;  // LMER_BINS L-mer frequencies
+  int P, rank;
+};
+
+struct OwnState {
+  uint32_t lm, min_lmer;
+  ull min_f;
+  int min_pos;
+};
+
+__device__ __forceinline__ int lmer_owner(uint32_t lmer, int P) {
+  const long long v = 68111LL * (long long)lmer + 105929LL;  // hasha * x + hashb
+  const long long h = ((v >> 31) + v) & 2147483647LL;
+  return (int)h % P;
+}
+
+// One byte of a read (index idx, 3-bit code el; km already updated).
+// Returns whether the k-mer ending here, if any, is owned.
+template <class Key, bool OWN>
+__device__ __forceinline__ bool own_step(OwnState& o, const OwnCfg& cf, Key km, uint32_t el, uint32_t idx,
+                                         int K) {
+  if (!OWN) return true;
+  o.lm = ((o.lm << 2) | el) & (LMER_BINS - 1);
+  if (idx < (uint32_t)(K - 1)) {
+    if (idx >= (uint32_t)(LMER_L - 1)) {
+      const ull f = cf.freq[o.lm];
+      if (idx == (uint32_t)(LMER_L - 1)) {
+        o.min_lmer = o.lm;
+        o.min_f = f;
+      } else if (f < o.min_f) {
+        o.min_lmer = o.lm;
+        o.min_f = f;
+        o.min_pos = (int)idx - (LMER_L - 1);
+      }
+    }
+    return false;
+  }
+  const ull f = cf.freq[o.lm];
+  if (o.min_pos < 0) {
+    for (int kk = 0; kk <= K - LMER_L; kk++) {
+      const uint32_t lo = (uint32_t)(km >> (2 * (K - kk - LMER_L))) & (LMER_BINS - 1);
+      const ull g = cf.freq[lo];
+      if (kk == 0 || g < o.min_f) {
+        o.min_lmer = lo;
+        o.min_f = g;
+        o.min_pos = kk;
+      }
+    }
+  }
+  if (f < o.min_f) {
+    o.min_lmer = o.lm;
+    o.min_f = f;
+    o.min_pos = K - LMER_L;
+  }
+  o.min_pos--;
+  return lmer_owner(o.min_lmer, cf.P) == cf.rank;
+}
+
+template <class Key, int ENC, bool OWN, class Emit>
 __device__ __forceinline__ int walk_read(const ChunkLds& C, const uint8_t* fa, ull s, uint32_t len, uint32_t& i,
-                                         Key& km, int K, Key mask, int Q, Emit&& emit) {
+                                         Key& km, OwnState& os, const OwnCfg& cf, int K, Key mask, int Q,
+                                         Emit&& emit) {
   int q = 0;
   if (C.lds) {
     // 8 staged bytes per LDS load
@@ -538,32 +610,51 @@ __device__ __forceinline__ int walk_read(const ChunkLds& C, const uint8_t* fa, u
       ull w = C.sb[off >> 3] >> ((off & 7) * 8);
       int nb = 8 - (int)(off & 7);
       while (nb > 0 && q < Q && i < len) {
-        km = ((km << 2) | (Key)base_code<ENC>((uint32_t)w & 0xFFu)) & mask;
+        const uint32_t el = base_code<ENC>((uint32_t)w & 0xFFu);
+        km = ((km << 2) | (Key)el) & mask;
+        const bool own = own_step<Key, OWN>(os, cf, km, el, i, K);
         w >>= 8;
         nb--;
         i++;
-        if (i >= (uint32_t)K) emit(q++, km);
+        if (i >= (uint32_t)K && own) emit(q++, km);
       }
     }
     return q;
   }
   while (q < Q && i < len) {
-    const uint32_t c = fa[s + i];
-    km = ((km << 2) | (Key)base_code<ENC>(c)) & mask;
+    const uint32_t el = base_code<ENC>(fa[s + i]);
+    km = ((km << 2) | (Key)el) & mask;
+    const bool own = own_step<Key, OWN>(os, cf, km, el, i, K);
     i++;
-    if (i >= (uint32_t)K) emit(q++, km);
+    if (i >= (uint32_t)K && own) emit(q++, km);
   }
   return q;
+}
+
+// L-mer frequencies (Sliding_window_l, kmer_counting.cpp:391-438): every
+// L-mer of every record's read line, rolling 3-bit update masked to 2L bits.
+__global__ void k_lmer_freq(const uint8_t* fa, const ull* rstart, const uint32_t* rlen, uint64_t R,
+                            ull* freq) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < R;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const ull s = rstart[r];
+    const uint32_t len = rlen[r];
+    uint32_t lm = 0;
+    for (uint32_t i = 0; i < len; i++) {
+      lm = ((lm << 2) | base_code<ENC_MAIN>(fa[s + i])) & (LMER_BINS - 1);
+      if (i >= (uint32_t)(LMER_L - 1)) atomicAdd(&freq[lm], 1ull);
+    }
+  }
 }
 
 // Level-1 histogram: digit = bits [KB - d1, KB) of each key (KB = key bits:
 // 2K on the main path, the whole word for the builder encoding).  Block b's
 // count of digit d goes to H[d * nblk + b] (digit-major), so an exclusive
 // scan of H gives every block the output base of each of its digit runs.
-template <class Key, int ENC>
+template <class Key, int ENC, bool OWN>
 __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n, const ull* rstart,
                                                    const uint32_t* rlen, uint64_t R, int K, int KB,
-                                                   Key mask, int d1, int rpb, uint32_t* H) {
+                                                   Key mask, int d1, int rpb, OwnCfg cf, uint32_t* H) {
   __shared__ ChunkLds C;
   __shared__ uint32_t h[256];
   const int nb = 1 << d1;
@@ -578,8 +669,9 @@ __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n
       const uint64_t r = c0 + threadIdx.x;
       uint32_t i = 0;
       Key km = 0;
-      walk_read<Key, ENC>(C, fa, rstart[r], rlen[r], i, km, K, mask, 0x7fffffff,
-                          [&](int, Key x) { atomicAdd(&h[key_digit<Key>(x, bpos, d1)], 1u); });
+      OwnState os{0, 0, 0, 0};
+      walk_read<Key, ENC, OWN>(C, fa, rstart[r], rlen[r], i, km, os, cf, K, mask, 0x7fffffff,
+                               [&](int, Key x) { atomicAdd(&h[key_digit<Key>(x, bpos, d1)], 1u); });
     }
     c0 += m;
     __syncthreads();
@@ -636,11 +728,12 @@ __device__ __forceinline__ void tile_split_write(TileLds<Key, NT>& L, int np, Ge
 // digit bases (scanned histogram O, same layout as H).
 // qt (<= Q) keys per thread per tile: chosen so that a typical read's
 // k-mers fill whole tiles (69 k-mers: 5 tiles of 14 rather than 16+16+16+16+5).
-template <class Key, int ENC>
+template <class Key, int ENC, bool OWN>
 __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_t n,
                                                       const ull* rstart, const uint32_t* rlen,
                                                       uint64_t R, int K, int KB, Key mask, int d1,
-                                                      int rpb, int qt, const ull* O, Key* S1) {
+                                                      int rpb, int qt, OwnCfg cf, const ull* O,
+                                                      Key* S1) {
   constexpr int Q = TileCfg<Key>::Q;
   __shared__ TileLds<Key, L1_NT> L;
   __shared__ ChunkLds C;
@@ -660,14 +753,15 @@ __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_
     const uint32_t len = mine ? rlen[c0 + threadIdx.x] : 0;
     uint32_t i = 0;
     Key km = 0;
+    OwnState os{0, 0, 0, 0};
     // Barriers per tile: the loop test, histogram done, two in the scan,
     // ranks ready, tile sorted.  Each thread resets its own histogram entry
     // after reading it; the loop test orders that before the next tile's
     // atomics, and the write-out's T1 reads before the next walk's stores.
     while (__syncthreads_or(i < len)) {
       Key* stage = L.T1 + threadIdx.x * Q;
-      const int np = walk_read<Key, ENC>(C, fa, s, len, i, km, K, mask, qt,
-                                         [&](int q, Key x) { stage[q] = x; });
+      const int np = walk_read<Key, ENC, OWN>(C, fa, s, len, i, km, os, cf, K, mask, qt,
+                                              [&](int q, Key x) { stage[q] = x; });
       Key kr[Q];
 #pragma unroll
       for (int q = 0; q < Q; q++) kr[q] = q < np ? stage[q] : Key(0);
@@ -1776,7 +1870,7 @@ enum BufId {
   B_FASTA, B_BLK, B_BLKOFF, B_NL, B_RSTART, B_RLEN, B_KC, B_KOFS, B_SMALL, B_BCNT, B_BOFF, B_BCUR,
   B_S, B_C, B_TU, B_UOFF, B_TK, B_TC, B_HIST, B_KBS, B_KBOFF, B_KK, B_KC2, B_MNQ, B_MNPLAN,
   B_MNNT, B_MNNG, B_MNNOFF, B_MNFLAG, B_NODES, B_TMP, B_OVF, B_PT0, B_PT1, B_PT2, B_PT3, B_PT4,
-  B_PT5, B_PT6, B_BLKGT, B_S2, B_BCNT2, B_BOFF2, B_BCUR2, B_TP, B_HIST1, B_OFF1, B_HIST2, B_OFF2, B_CUTBH, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
+  B_PT5, B_PT6, B_BLKGT, B_S2, B_BCNT2, B_BOFF2, B_BCUR2, B_TP, B_HIST1, B_OFF1, B_HIST2, B_OFF2, B_CUTBH, B_LFREQ, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
 };
 
 }  // namespace
@@ -2153,9 +2247,9 @@ int records_builder(pk_ctx* c, const uint8_t* fa, uint64_t n, int klen, uint64_t
 // order), N instances.
 // K = window length, KB = key bits (2K on the main path, the whole word for
 // the builder encoding), mask = word mask of the rolling update.
-template <class Key, int ENC>
+template <class Key, int ENC, bool OWN = false>
 int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, const uint32_t* rlen,
-                  uint64_t R, uint64_t N, int K, int KB, Key mask) {
+                  uint64_t R, uint64_t N, int K, int KB, Key mask, OwnCfg cf = OwnCfg{nullptr, 1, 0}) {
   constexpr int CAP = HashCfg<Key>::LIMIT;
   hipStream_t st = c->stream;
   ull* small = reinterpret_cast<ull*>(c->buf[B_SMALL].p);
@@ -2178,7 +2272,8 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   // 3. partition: level 1 fused with the parse, further levels from memory,
   // at most 8 bits per level, until a bucket holds ~1.5 hash tables' worth
   // of instances.  Write bases come from scanned digit-major histograms.
-  const int btot = choose_bits(N, KB / 2, CAP);
+  // with owner routing, about N / P instances are this rank's
+  const int btot = choose_bits(OWN ? (N + cf.P - 1) / cf.P : N, KB / 2, CAP);
   const int d1 = btot < 8 ? btot : 8;
   const uint64_t B1 = 1ull << d1;
   // reads per block: up to 8 chunks of 256, fewer for small inputs so that
@@ -2203,17 +2298,32 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   PK_TRY(need(c, B_C, N, &C));
   c->timing.bucket_bits = btot;
   PK_HIP(c, hipEventRecord(c->ev[2], st));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_hist<Key, ENC>), dim3((unsigned)nblk), dim3(L1_NT), 0, st, fa,
-                     n, rstart, rlen, R, K, KB, mask, d1, rpb, H);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_hist<Key, ENC, OWN>), dim3((unsigned)nblk), dim3(L1_NT), 0, st,
+                     fa, n, rstart, rlen, R, K, KB, mask, d1, rpb, cf, H);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[3], st));
   PK_TRY(scan_excl<uint32_t>(c, H, O, B1 * nblk));
+  if (OWN) {
+    // this rank's instances
+    ull no = 0;
+    PK_TRY(d2h(c, &no, O + B1 * nblk, 8));
+    N = no;
+    c->N = N;
+    c->timing.n_instances = N;
+    if (N == 0) {
+      c->U = 0;
+      PK_TRY(need(c, B_TK, 1, &TK));
+      PK_TRY(need(c, B_TC, 1, &TC));
+      c->have_table = true;
+      return PK_OK;
+    }
+  }
   hipLaunchKernelGGL(k_l1_boff, dim3(grid_for(B1 + 1, 256)), dim3(256), 0, st, O, (int)B1, nblk, (ull)N,
                      boff);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[4], st));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key, ENC>), dim3((unsigned)nblk), dim3(L1_NT), 0, st,
-                     fa, n, rstart, rlen, R, K, KB, mask, d1, rpb, qt, O, S);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key, ENC, OWN>), dim3((unsigned)nblk), dim3(L1_NT), 0,
+                     st, fa, n, rstart, rlen, R, K, KB, mask, d1, rpb, qt, cf, O, S);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[16], st));
   uint64_t B = B1;
@@ -2365,6 +2475,43 @@ int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
   uint64_t R, N;
   PK_TRY(records_main(c, fa, n, K, &rstart, &rlen, &R, &N));
   return count_records<Key, ENC_MAIN>(c, fa, n, rstart, rlen, R, N, K, 2 * K, key_mask<Key>(2 * K));
+}
+
+// L-mer frequencies of the whole input into freq (device, LMER_BINS u64),
+// with the record rule of Sliding_window_l (a record needs L bytes).
+int lmer_frequency_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, ull** freq_out) {
+  ull* rs;
+  uint32_t* rl;
+  uint64_t R, N;
+  PK_TRY(records_main(c, fa, n, LMER_L, &rs, &rl, &R, &N));
+  ull* freq;
+  PK_TRY(need(c, B_LFREQ, LMER_BINS, &freq));
+  PK_TRY(fill_u64(c, freq, 0, LMER_BINS));
+  if (R) {
+    hipLaunchKernelGGL(k_lmer_freq, dim3((unsigned)std::min<uint64_t>(grid_for(R, 256), 4096)), dim3(256), 0,
+                       c->stream, fa, rs, rl, R, freq);
+    PK_HIP(c, hipGetLastError());
+  }
+  *freq_out = freq;
+  return PK_OK;
+}
+
+// One rank's share of the counted table at P ranks (the table rank `rank`
+// holds after the reference's all-to-all): the k-mers whose owner is rank.
+template <class Key>
+int rank_share_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K, int P, int rank) {
+  c->have_table = c->have_kept = c->have_nodes = false;
+  c->timing = pk_timing{};
+  c->key_bits = 2 * K;
+  PK_HIP(c, hipEventRecord(c->ev[0], c->stream));
+  ull* freq;
+  PK_TRY(lmer_frequency_impl(c, fa, n, &freq));
+  ull* rstart;
+  uint32_t* rlen;
+  uint64_t R, N;
+  PK_TRY(records_main(c, fa, n, K, &rstart, &rlen, &R, &N));
+  return count_records<Key, ENC_MAIN, true>(c, fa, n, rstart, rlen, R, N, K, 2 * K, key_mask<Key>(2 * K),
+                                            OwnCfg{freq, P, rank});
 }
 
 template <class Key>
@@ -2592,6 +2739,50 @@ int pk_count(pk_ctx* c, const char* fasta, uint64_t nbytes, int k, int fasta_on_
   out->keys = reinterpret_cast<const uint64_t*>(c->buf[B_TK].p);
   out->counts = reinterpret_cast<const int32_t*>(c->buf[B_TC].p);
   return PK_OK;
+}
+
+static int fasta_to_device(pk_ctx* c, const char* fasta, uint64_t nbytes, int fasta_on_device,
+                           const uint8_t** fa) {
+  if (fasta_on_device) {
+    *fa = reinterpret_cast<const uint8_t*>(fasta);
+    return PK_OK;
+  }
+  uint8_t* d;
+  PK_TRY(need(c, B_FASTA, nbytes + 64, &d));
+  if (nbytes) PK_HIP(c, hipMemcpyAsync(d, fasta, nbytes, hipMemcpyHostToDevice, c->stream));
+  *fa = d;
+  return PK_OK;
+}
+
+int pk_count_rank_share(pk_ctx* c, const char* fasta, uint64_t nbytes, int k, int nranks, int rank,
+                        int fasta_on_device, pk_table* out) {
+  if (!c || !out || (!fasta && nbytes)) return PK_EINVAL;
+  if (k < 8 || k > 64) return c->fail(PK_EINVAL, "k must be in [8, 64]");
+  if (nranks < 1 || rank < 0 || rank >= nranks) return c->fail(PK_EINVAL, "need 0 <= rank < nranks");
+  if (nranks == 1) return pk_count(c, fasta, nbytes, k, fasta_on_device, out);
+  PK_HIP(c, hipSetDevice(c->device));
+  const uint8_t* fa;
+  PK_TRY(fasta_to_device(c, fasta, nbytes, fasta_on_device, &fa));
+  int rc = (k <= 32) ? rank_share_impl<uint64_t>(c, fa, nbytes, k, nranks, rank)
+                     : rank_share_impl<u128>(c, fa, nbytes, k, nranks, rank);
+  if (rc) return rc;
+  out->n = c->U;
+  out->words = c->words;
+  out->k = c->k;
+  out->keys = reinterpret_cast<const uint64_t*>(c->buf[B_TK].p);
+  out->counts = reinterpret_cast<const int32_t*>(c->buf[B_TC].p);
+  return PK_OK;
+}
+
+int pk_lmer_frequency(pk_ctx* c, const char* fasta, uint64_t nbytes, int fasta_on_device,
+                      uint64_t* freq_host) {
+  if (!c || !freq_host || (!fasta && nbytes)) return PK_EINVAL;
+  PK_HIP(c, hipSetDevice(c->device));
+  const uint8_t* fa;
+  PK_TRY(fasta_to_device(c, fasta, nbytes, fasta_on_device, &fa));
+  ull* freq;
+  PK_TRY(lmer_frequency_impl(c, fa, nbytes, &freq));
+  return d2h(c, freq_host, freq, LMER_BINS * 8);
 }
 
 int pk_builder_count(pk_ctx* c, const char* fasta, uint64_t nbytes, int klen, uint64_t read_len,

@@ -208,3 +208,39 @@ def test_bacterial_full_size_properties(ctx, gen_mod):
     assert int((nodes["suf_cnt"] > 0).sum()) == nk and int((nodes["pre_cnt"] > 0).sum()) == nk
     assert int(nodes["suf_cnt"].astype(np.int64).sum()) == int(kc.astype(np.int64).sum())
     assert np.all(nodes["key_lo"][1:] > nodes["key_lo"][:-1])
+
+
+NON_ACGT = (b">h\n" + b"ACGTNNacgtRYKMACGTTGCAAGGCTTAACCGGTTAA" * 6 + b"\n>h2\n"
+            + b"nNnNACGTTTGGCCAAGGTTCCAAGGTTACGATCGATCGGATCAGCTAGC" * 4 + b"\n")
+
+
+@pytest.mark.parametrize("name", ["syn_tiny.fa.gz", "syn_mid.fa.gz"])
+def test_lmer_frequency(ctx, oracle_mod, name):
+    _p(f"case: lmer frequency {name}")
+    fa = read_fasta(name)
+    assert np.array_equal(ctx.lmer_frequency(fa), oracle_mod.lmer_frequency(fa))
+    assert np.array_equal(ctx.lmer_frequency(NON_ACGT), oracle_mod.lmer_frequency(NON_ACGT))
+
+
+@pytest.mark.parametrize("k,P", [(32, 2), (32, 3), (21, 8), (64, 3), (45, 2)])
+def test_rank_share(ctx, oracle_mod, k, P):
+    """One rank's share (owner = hash31(min-frequency 8-mer) % P) equals the
+    oracle's, and on ACGT-only reads the shares partition the whole table."""
+    _p(f"case: rank share k={k} P={P}")
+    for name in ("syn_mid.fa.gz", None):
+        fa = read_fasta(name) if name else NON_ACGT
+        total = 0
+        for r in range(P):
+            ctx.count_rank_share(fa, k, P, r)
+            keys, counts = ctx.table_host()
+            ok, oc = oracle_mod.count(fa, k, nranks=P, rank=r)
+            assert np.array_equal(keys, ok) and np.array_equal(counts, oc), (name, r)
+            total += keys.shape[0]
+        ctx.count(fa, k)
+        if name:
+            assert total == ctx.table.n
+        else:
+            # with non-ACGT bytes the owner also depends on bases outside the
+            # k-mer (the 3-bit code of the rolling 8-mer), so one k-mer value
+            # can reach two ranks -- the oracle agrees rank by rank
+            assert total >= ctx.table.n

@@ -266,3 +266,48 @@ def test_builder_closed_form_detects_planted_mistakes():
     # the first '>' of a sequence line starts a record
     r3, _ = builder_closed_form(fa.replace(b"seq>with>gt", b"seq with gt"), 21, 100, 64)
     assert r3 == reads - 1
+
+
+def test_rank_share_owner_closed_form(oracle_mod):
+    """On ACGT-only reads the owner the oracle tracks incrementally equals the
+    closed form: hash31 of the leftmost minimum-frequency 8-mer of the k-mer
+    (frequencies = numpy count of every 8-mer of every read), modulo P."""
+    from collections import Counter
+    from numpy.lib.stride_tricks import sliding_window_view
+    fa = read_fasta("syn_tiny.fa.gz")
+    k, L, P = 32, 8, 3
+    code = np.zeros(256, dtype=np.uint64)
+    for ch, v in zip(b"ACTG", range(4)):
+        code[ch] = v
+    reads = [np.frombuffer(ln, dtype=np.uint8) for ln in fa.split(b"\n")[1::2] if len(ln) >= L]
+    lm_all = []
+    per_read = []
+    for r in reads:
+        c = code[r]
+        lm = np.zeros(len(c) - L + 1, dtype=np.uint64)
+        for j in range(L):
+            lm = (lm << np.uint64(2)) | c[j:len(c) - L + 1 + j]
+        lm_all.append(lm)
+        per_read.append((c, lm))
+    freq = np.bincount(np.concatenate(lm_all).astype(np.int64), minlength=1 << 16)
+    assert np.array_equal(freq.astype(np.uint64), oracle_mod.lmer_frequency(fa))
+    a, b, M = 68111, 105929, (1 << 31) - 1
+    hv = a * np.arange(1 << 16, dtype=np.int64) + b
+    owner_of = ((hv >> 31) + hv) & M
+    want = [Counter() for _ in range(P)]
+    for c, lm in per_read:
+        if len(c) < k:
+            continue
+        win = sliding_window_view(freq[lm.astype(np.int64)], k - L + 1)  # per k-mer
+        first = np.argmin(win, axis=1)                                  # leftmost minimum
+        mins = lm[np.arange(win.shape[0]) + first]
+        own = owner_of[mins.astype(np.int64)] % P
+        km = np.zeros(len(c) - k + 1, dtype=np.uint64)
+        for j in range(k):
+            km = (km << np.uint64(2)) | c[j:len(c) - k + 1 + j]
+        for x, o in zip(km.tolist(), own.tolist()):
+            want[o][x] += 1
+    for r in range(P):
+        kr, cr = oracle_mod.count(fa, k, nranks=P, rank=r)
+        got = Counter({int(x): int(y) for x, y in zip(kr[:, 0], cr)})
+        assert got == want[r], r
