@@ -13,11 +13,17 @@
 //       construct_kmer_histogram, src/kmer_histogram.cpp:72;
 //       construct_macro_nodes, src/mn_construct.cpp:212) as value-returning
 //       calls on one device context.
+//   pakhip::visit_value / pakhip::reference_prefix_order
+//       the per-edge visit count and the reference's prefix order, for
+//       callers that fill the reference's MacroNode records.
 //
 // Header-only; link with -lpakhip.  Host code only (no HIP headers needed).
 #ifndef PAKHIP_BUILDERS_HPP
 #define PAKHIP_BUILDERS_HPP
 
+#include <algorithm>
+#include <array>
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <limits>
@@ -198,6 +204,60 @@ inline std::vector<pk_mnode> macro_nodes(pk_ctx* ctx) {
   std::vector<pk_mnode> nodes((size_t)m.n);
   if (m.n) throw_on_error(ctx, pk_download_nodes(ctx, nodes.data()), "pk_download_nodes");
   return nodes;
+}
+
+// visit_value (src/mn_construct.cpp:75-79): the visit count of an edge of
+// `count` instances at `coverage`, computed as the reference does (double
+// divide, ceil).
+inline int visit_value(int count, int coverage) {
+  return (int)std::ceil((double)count / (double)coverage);
+}
+
+// Reference prefix order (src/mn_construct.cpp:87-95,299-303,395-457): the
+// reference walks a copy of the kept list sorted by the suffix key only
+// (Comp_suffix), so a node's prefixes come in the order std::sort leaves
+// tied elements.  This repeats that sort (same algorithm, same input order:
+// the kept list ascending, same comparisons) and returns, for every node of
+// `nodes` (ascending keys), its prefix bases in the reference's order (255 =
+// unused slot).  Exact against a one-thread reference run built with the
+// same libstdc++; the device's records carry the counts by base.
+inline std::vector<std::array<uint8_t, 4>> reference_prefix_order(const Kmer_Table& kept,
+                                                                  const std::vector<pk_mnode>& nodes) {
+  typedef unsigned __int128 u128;
+  const int k = kept.k;
+  const size_t n = kept.size();
+  std::vector<u128> seq(n);
+  for (size_t i = 0; i < n; i++)
+    seq[i] = kept.words == 1 ? (u128)kept.keys[i]
+                             : (((u128)kept.keys[2 * i + 1] << 64) | (u128)kept.keys[2 * i]);
+  const u128 succ_mask = (((u128)1) << (2 * (k - 1))) - 1;
+  if (kept.words == 1) {
+    std::vector<uint64_t> s64(n);
+    for (size_t i = 0; i < n; i++) s64[i] = (uint64_t)seq[i];
+    const uint64_t m = (uint64_t)succ_mask;
+    std::sort(s64.begin(), s64.end(), [m](uint64_t a, uint64_t b) { return (a & m) < (b & m); });
+    for (size_t i = 0; i < n; i++) seq[i] = s64[i];
+  } else {
+    std::sort(seq.begin(), seq.end(),
+              [succ_mask](u128 a, u128 b) { return (a & succ_mask) < (b & succ_mask); });
+  }
+  std::vector<std::array<uint8_t, 4>> order(nodes.size());
+  for (auto& o : order) o = {255, 255, 255, 255};
+  size_t nd = 0;
+  for (size_t i = 0; i < n; i++) {
+    const u128 succ = seq[i] & succ_mask;
+    while (nd < nodes.size() &&
+           ((((u128)nodes[nd].key_hi) << 64) | (u128)nodes[nd].key_lo) < succ)
+      nd++;
+    if (nd == nodes.size()) break;
+    const uint8_t base = (uint8_t)((seq[i] >> (2 * (k - 1))) & 3);
+    for (int j = 0; j < 4; j++)
+      if (order[nd][j] == 255) {
+        order[nd][j] = base;
+        break;
+      }
+  }
+  return order;
 }
 
 }  // namespace pakhip

@@ -71,13 +71,26 @@ def build_adapter_test() -> str:
     return out
 
 
+def build_prefix_order_test() -> str:
+    """Host-only driver of pakhip::reference_prefix_order (CPU tests)."""
+    src = os.path.join(ROOT, "tests", "cpp", "prefix_order_test.cpp")
+    outdir = os.path.join(ROOT, "tests", "cpp", "_bin")
+    os.makedirs(outdir, exist_ok=True)
+    out = os.path.join(outdir, "prefix_order_test")
+    _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", src, "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all() -> None:
     build_gen()
     build_hip()
     build_adapter_test()
+    build_prefix_order_test()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "gen": build_gen, "hip": build_hip, "adapter": build_adapter_test}[w]()
+        {"all": build_all, "gen": build_gen, "hip": build_hip, "adapter": build_adapter_test,
+         "prefix": build_prefix_order_test}[w]()
