@@ -93,9 +93,12 @@ def make_batched_sets(num_sets, threads=8):
     return specs, fas
 
 
-def run_plain_job(ctx, dev, k, t):
-    """One job on a plain context, synchronous: (seconds, instances, U, V, M)."""
-    from pakman_plus_amd import api
+def run_plain_job(ctx, dev, k, t, table=None):
+    """One job on a plain context, synchronous: (seconds, instances, U, V, M).
+    With a tuning table the job's size entry is applied first (as the
+    server does per job)."""
+    from pakman_plus_amd import api, tune
+    ctx.set_tuning(**tune.for_size(table or [], int(dev.numel())))
     t0 = time.perf_counter()
     ctx.count(None, k, device_ptr=dev.data_ptr(), nbytes=dev.numel())
     U = ctx.table.n
@@ -130,14 +133,17 @@ def bench_batched_served(args):
     # -- plain, one job at a time (k = 32) -------------------------------------
     # warm-up grows the buffers to the largest job (u128 keys at k = 64)
     big = sorted(range(len(devs)), key=lambda i: -devs[i].numel())
+    from pakman_plus_amd import tune
+    table = tune.load()
     ctx = api.Context(0)
     run_plain_job(ctx, devs[big[0]], 64, args.t)
-    plain32 = [run_plain_job(ctx, d, 32, args.t) for d in devs]
+    plain32 = [run_plain_job(ctx, d, 32, args.t, table) for d in devs]
     plain_s = sum(p[0] for p in plain32)
     N = sum(p[1] for p in plain32)
     alg = sum(alg_bytes(p[1], p[2], p[3], p[4], int(d.numel()), W)["pipeline"] for p, d in zip(plain32, devs))
     # -- batched ------------------------------------------------------------------
     srv = api.Server(0, n_slots=args.slots)
+    srv.set_tuning(table)
     jobs = [{"k": 32, "t": args.t, "want_nodes": True, "device_ptr": d.data_ptr(), "nbytes": d.numel()}
             for d in devs]
     warm = srv.submit_batch([dict(jobs[i], k=64) for i in big[: args.slots]])
@@ -159,12 +165,14 @@ def bench_batched_served(args):
         "plain_value": N / plain_s, "plain_seconds": plain_s, "speedup_vs_plain": plain_s / batched_s,
         "hbm_frac": alg / batched_s / HBM_PEAK, "plain_hbm_frac": alg / plain_s / HBM_PEAK,
         "results_match_plain": same, "generation_s": t_gen,
+        "tuning": "autotuned per size (pakman_plus_amd/tuning.json), same table for plain and batched"
+                  if table else "built-in",
     }
     # -- served: Poisson arrivals, k from {32, 64} ---------------------------------
     rng = np.random.default_rng(5003)
     order = rng.permutation(len(devs))
     ks = rng.choice([32, 64], size=len(devs))
-    plain64 = {int(i): run_plain_job(ctx, devs[i], 64, args.t) for i in order if ks[i] == 64}
+    plain64 = {int(i): run_plain_job(ctx, devs[i], 64, args.t, table) for i in order if ks[i] == 64}
     service = np.array([plain64[int(i)][0] if ks[i] == 64 else plain32[int(i)][0] for i in order])
     inst = np.array([plain64[int(i)][1] if ks[i] == 64 else plain32[int(i)][1] for i in order])
     lam = args.load * len(order) / service.sum()   # requests per second
@@ -271,6 +279,11 @@ def main():
     dev = torch.from_numpy(fa).to("cuda:0")
     torch.cuda.synchronize()
     ctx = api.Context(0)
+    from pakman_plus_amd import tune
+    tuning_table = tune.load()
+    tun = tune.for_size(tuning_table, nbytes)
+    if tun:
+        ctx.set_tuning(**tun)
 
     def step():
         ctx.count(None, k, device_ptr=dev.data_ptr(), nbytes=nbytes)
@@ -323,7 +336,8 @@ def main():
         "config": {"workload": desc, "k": k, "t": args.t, "reads": int(tm["n_reads"]),
                    "kmer_instances": N, "unique_kmers": U, "kept_kmers": V, "macro_nodes": M,
                    "fasta_bytes": nbytes, "bucket_bits": int(tm["bucket_bits"]),
-                   "split_buckets": int(tm["n_split"]), "parallelism": "1 GPU"},
+                   "split_buckets": int(tm["n_split"]), "parallelism": "1 GPU",
+                   "tuning": tun or "built-in"},
         "roofline": {"bound": "hbm", "kernel": KERNELS[dom].format(key="unsigned long" if k <= 32 else "u128"),
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,

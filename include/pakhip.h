@@ -102,6 +102,21 @@ const char* pk_last_error(const pk_ctx* ctx);
 void* pk_ctx_stream(pk_ctx* ctx);
 /* Device bytes currently held by the context. */
 uint64_t pk_ctx_device_bytes(const pk_ctx* ctx);
+/* Launch-shape choices of the counting kernels (results do not depend on
+ * them).  Zero fields keep the built-in choice.
+ *   l1_chunks      reads per level-1 partition block, in chunks of 256 (1..64;
+ *                  built-in: 1..8 by input size)
+ *   l1_bits        level-1 radix digit width (1..8; built-in 8)
+ *   bucket_factor  instances per final bucket as a multiple of the LDS hash
+ *                  table's key capacity, i.e. the inverse hash load target
+ *                  (built-in 3.0) */
+typedef struct pk_tuning {
+  int32_t l1_chunks;
+  int32_t l1_bits;
+  float bucket_factor;
+} pk_tuning;
+int pk_ctx_set_tuning(pk_ctx* ctx, const pk_tuning* t);
+
 /* Free every device buffer of the context (they grow back on the next call);
  * invalidates the table, kept table and node pointers. */
 int pk_ctx_release_buffers(pk_ctx* ctx);
@@ -223,6 +238,15 @@ int pk_server_poll(pk_server* srv, uint64_t ticket, pk_job_result* out);
 int pk_server_download(pk_server* srv, uint64_t ticket, uint64_t* keys, int32_t* counts,
                        uint64_t* kept_keys, int32_t* kept_counts, uint64_t* hist_t,
                        pk_mnode* nodes);
+/* Per-size tuning for the jobs of a server: a job of nbytes uses the first
+ * entry (ascending max_bytes) with nbytes <= max_bytes, else the built-in
+ * choices.  Copies the table; n == 0 clears it. */
+typedef struct pk_tune_entry {
+  uint64_t max_bytes;
+  pk_tuning tuning;
+} pk_tune_entry;
+int pk_server_set_tuning(pk_server* srv, const pk_tune_entry* entries, int n);
+
 /* Free a finished job's device results and forget the ticket. */
 int pk_server_release(pk_server* srv, uint64_t ticket);
 const char* pk_server_last_error(const pk_server* srv);

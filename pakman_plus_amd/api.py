@@ -50,6 +50,16 @@ class _Timing(ctypes.Structure):
                 ("l1_scatter_ms", ctypes.c_float)]
 
 
+class Tuning(ctypes.Structure):
+    """Launch-shape choices of the counting kernels (pk_tuning); 0 = built-in."""
+    _fields_ = [("l1_chunks", ctypes.c_int32), ("l1_bits", ctypes.c_int32),
+                ("bucket_factor", ctypes.c_float)]
+
+
+class _TuneEntry(ctypes.Structure):
+    _fields_ = [("max_bytes", ctypes.c_uint64), ("tuning", Tuning)]
+
+
 class _Job(ctypes.Structure):
     _fields_ = [("fasta", ctypes.c_void_p), ("nbytes", ctypes.c_uint64), ("k", ctypes.c_int32),
                 ("t", ctypes.c_int32), ("fasta_on_device", ctypes.c_int32),
@@ -70,12 +80,14 @@ MNODE_DTYPE = np.dtype([("key_lo", "<u8"), ("key_hi", "<u8"), ("pre_cnt", "<i4",
 
 # every symbol declared in include/pakhip.h
 EXPORTS = ("pk_ctx_create", "pk_ctx_destroy", "pk_last_error", "pk_ctx_stream",
-           "pk_ctx_device_bytes", "pk_ctx_release_buffers", "pk_count", "pk_count_rank_share",
+           "pk_ctx_device_bytes", "pk_ctx_set_tuning", "pk_ctx_release_buffers", "pk_count",
+           "pk_count_rank_share",
            "pk_lmer_frequency", "pk_builder_count",
            "pk_cutoff", "pk_kept_table", "pk_macro_nodes", "pk_download_table", "pk_download_kept",
            "pk_download_nodes", "pk_last_timing", "pk_server_create", "pk_server_destroy",
            "pk_server_now", "pk_server_submit", "pk_server_submit_batch", "pk_server_wait",
-           "pk_server_poll", "pk_server_download", "pk_server_release", "pk_server_last_error")
+           "pk_server_poll", "pk_server_download", "pk_server_set_tuning", "pk_server_release",
+           "pk_server_last_error")
 
 _LIB = None
 
@@ -129,6 +141,10 @@ def load_library():
     L.pk_download_nodes.restype = i32
     L.pk_last_timing.argtypes = [vp, ctypes.POINTER(_Timing)]
     L.pk_last_timing.restype = i32
+    L.pk_ctx_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    L.pk_ctx_set_tuning.restype = i32
+    L.pk_server_set_tuning.argtypes = [vp, ctypes.POINTER(_TuneEntry), i32]
+    L.pk_server_set_tuning.restype = i32
     L.pk_ctx_release_buffers.argtypes = [vp]
     L.pk_ctx_release_buffers.restype = i32
     L.pk_server_create.argtypes = [ctypes.POINTER(vp), i32, i32, u64]
@@ -214,6 +230,10 @@ class Context:
     @property
     def device_bytes(self) -> int:
         return int(self._lib.pk_ctx_device_bytes(self._h))
+
+    def set_tuning(self, l1_chunks: int = 0, l1_bits: int = 0, bucket_factor: float = 0.0):
+        t = Tuning(l1_chunks, l1_bits, bucket_factor)
+        self._check(self._lib.pk_ctx_set_tuning(self._h, ctypes.byref(t)))
 
     # -- perform_kmer_counting ------------------------------------------------
     def count(self, fasta, k: int, device_ptr: int | None = None, nbytes: int | None = None) -> TableView:
@@ -465,6 +485,15 @@ class Server:
     def release(self, ticket: int):
         self._check(self._lib.pk_server_release(self._h, ticket))
         self._t.pop(ticket, None)
+
+    def set_tuning(self, entries: list[dict]):
+        """entries: dicts with max_bytes, l1_chunks, l1_bits, bucket_factor
+        (the autotuner's table, pakman_plus_amd.tune)."""
+        arr = (_TuneEntry * max(len(entries), 1))()
+        for i, e in enumerate(entries):
+            arr[i] = _TuneEntry(int(e["max_bytes"]), Tuning(int(e.get("l1_chunks", 0)), int(e.get("l1_bits", 0)),
+                                                             float(e.get("bucket_factor", 0.0))))
+        self._check(self._lib.pk_server_set_tuning(self._h, arr, len(entries)))
 
 
 def nodes_as_arrays(nodes: np.ndarray, k: int) -> dict:

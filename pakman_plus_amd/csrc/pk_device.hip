@@ -1891,6 +1891,7 @@ struct pk_ctx {
   bool have_table = false, have_kept = false, have_nodes = false;
   int t = 0, mb = 0;
   int key_bits = 0;  // bits of a table key: 2k (main path) or the word (builder)
+  pk_tuning tuning{};  // zeros: built-in choices
 
   int fail(int code, const std::string& what) {
     err = what;
@@ -1991,9 +1992,9 @@ int d2h(pk_ctx* c, void* dst, const void* src, size_t bytes) {
 // Bucket bits: at least 1 (the hash table's EMPTY value flips the top key
 // bit), and enough that a bucket holds about 3 tables of instances (a bucket
 // with more distinct keys than a table holds takes extra passes).
-int choose_bits(uint64_t N, int K, int cap_keys) {
+int choose_bits(uint64_t N, int K, int cap_keys, float factor) {
   int b = 1;
-  const uint64_t target = (uint64_t)cap_keys * 3;
+  const uint64_t target = (uint64_t)((double)cap_keys * factor);
   while (b < 24 && b < 2 * K && (N >> b) > target) b++;
   return b;
 }
@@ -2273,12 +2274,17 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   // at most 8 bits per level, until a bucket holds ~1.5 hash tables' worth
   // of instances.  Write bases come from scanned digit-major histograms.
   // with owner routing, about N / P instances are this rank's
-  const int btot = choose_bits(OWN ? (N + cf.P - 1) / cf.P : N, KB / 2, CAP);
-  const int d1 = btot < 8 ? btot : 8;
+  const pk_tuning& tun = c->tuning;
+  const float bf = tun.bucket_factor > 0.f ? tun.bucket_factor : 3.0f;
+  const int btot = choose_bits(OWN ? (N + cf.P - 1) / cf.P : N, KB / 2, CAP, bf);
+  const int l1max = tun.l1_bits > 0 && tun.l1_bits < 8 ? tun.l1_bits : 8;
+  const int d1 = btot < l1max ? btot : l1max;
   const uint64_t B1 = 1ull << d1;
   // reads per block: up to 8 chunks of 256, fewer for small inputs so that
   // the grid still spreads over the CUs
-  const int rpb = L1_NT * (int)std::min<uint64_t>(8, std::max<uint64_t>(1, R / (L1_NT * 1024ull)));
+  const int rpb = L1_NT * (tun.l1_chunks > 0 && tun.l1_chunks <= 64
+                               ? tun.l1_chunks
+                               : (int)std::min<uint64_t>(8, std::max<uint64_t>(1, R / (L1_NT * 1024ull))));
   const uint64_t nblk = (R + rpb - 1) / rpb;
   int qt = TileCfg<Key>::Q;
   {
@@ -2347,8 +2353,10 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
     PK_TRY(need(c, o ? B_OFF2 : B_OFF1, (tiles << db) + 1, &O2));
     hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(1024), 0, st, boff, (int)B, TQ, tp);
     PK_HIP(c, hipGetLastError());
-    ull ntiles = 0;
-    PK_TRY(d2h(c, &ntiles, tp + B, 8));
+    // grid and scan over the upper bound of the level's tiles: blocks past
+    // tp[B] return at once, and the scan of their (unused) histogram slots
+    // only extends past every offset that is read
+    const ull ntiles = tiles;
     const int dpos = KB - fixed - db;
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_hist<Key>), dim3((unsigned)ntiles), dim3(MS_NT), 0, st, Sin,
                        boff, tp, (int)B, dpos, db, H2);
@@ -2702,6 +2710,19 @@ void pk_ctx_destroy(pk_ctx* c) {
 const char* pk_last_error(const pk_ctx* c) { return c ? c->err.c_str() : "null context"; }
 void* pk_ctx_stream(pk_ctx* c) { return c ? (void*)c->stream : nullptr; }
 uint64_t pk_ctx_device_bytes(const pk_ctx* c) { return c ? c->held : 0; }
+
+int pk_ctx_set_tuning(pk_ctx* c, const pk_tuning* t) {
+  if (!c) return PK_EINVAL;
+  if (!t) {
+    c->tuning = pk_tuning{};
+    return PK_OK;
+  }
+  if (t->l1_chunks < 0 || t->l1_chunks > 64 || t->l1_bits < 0 || t->l1_bits > 8 ||
+      t->bucket_factor < 0.f || t->bucket_factor > 64.f)
+    return c->fail(PK_EINVAL, "tuning values out of range");
+  c->tuning = *t;
+  return PK_OK;
+}
 
 int pk_ctx_release_buffers(pk_ctx* c) {
   if (!c) return PK_EINVAL;

@@ -72,6 +72,7 @@ struct pk_server {
   int running = 0;
   bool stop = false;
   std::string err;
+  std::vector<pk_tune_entry> tune;  // ascending max_bytes
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
   double now() const {
@@ -97,6 +98,16 @@ void pk_server::run(int slot, Job* j) {
     r.status = rc;
     std::snprintf(r.message, sizeof(r.message), "%s", pk_last_error(c));
   };
+  pk_tuning tu{};
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    for (const auto& e : tune)
+      if (j->spec.nbytes <= e.max_bytes) {
+        tu = e.tuning;
+        break;
+      }
+  }
+  pk_ctx_set_tuning(c, &tu);
   pk_table t{};
   int rc = pk_count(c, j->spec.fasta, j->spec.nbytes, j->spec.k, j->spec.fasta_on_device, &t);
   pk_timing tm{};
@@ -356,6 +367,16 @@ int pk_server_download(pk_server* s, uint64_t ticket, uint64_t* keys, int32_t* c
   if (e == hipSuccess) e = get(nodes, r.nodes.nodes, r.nodes.n * sizeof(pk_mnode));
   if (e != hipSuccess) return s->fail(PK_EHIP, std::string("download failed: ") + hipGetErrorString(e));
   if (hist_t && !j->hist.empty()) std::memcpy(hist_t, j->hist.data(), j->hist.size() * 8);
+  return PK_OK;
+}
+
+int pk_server_set_tuning(pk_server* s, const pk_tune_entry* entries, int n) {
+  if (!s || n < 0 || (n && !entries)) return PK_EINVAL;
+  std::vector<pk_tune_entry> t(entries, entries + n);
+  std::sort(t.begin(), t.end(),
+            [](const pk_tune_entry& a, const pk_tune_entry& b) { return a.max_bytes < b.max_bytes; });
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->tune = std::move(t);
   return PK_OK;
 }
 

@@ -1,0 +1,115 @@
+"""Autotuner of the counting kernels' launch shapes, per input-size bucket.
+
+For every size bucket (FASTA bytes up to max_bytes) it builds a read set of
+that size shaped like the batched workload (uniform genome, 100 bp reads,
+0.5% errors), times count + cutoff + macro nodes on the device for each
+candidate (reads per level-1 block, level-1 digit bits, bucket factor = the
+inverse of the hash-table load target) and keeps the fastest.  The table is
+written as JSON (pakman_plus_amd/tuning.json) and applied by Server.set_tuning
+or per context with Context.set_tuning.  Results never depend on the tuning.
+
+  python -m pakman_plus_amd.tune [--out PATH] [--reps 5]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import time
+
+DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+# bucket upper bounds (FASTA bytes); the last one catches everything larger
+BUCKETS = [1 << 21, 1 << 23, 1 << 25, 1 << 27, 1 << 29, 1 << 62]
+CANDIDATES = ([{"l1_chunks": c, "l1_bits": 8, "bucket_factor": f}
+               for c in (1, 2, 4, 8) for f in (1.5, 3.0, 5.0)]
+              + [{"l1_chunks": 0, "l1_bits": b, "bucket_factor": 3.0} for b in (6, 7)])
+
+
+def _sample(max_bytes: int, prev: int):
+    """A read set near the geometric middle of the bucket (prev, max_bytes]."""
+    from pakman_plus_amd import gen
+    lo = prev if prev else max_bytes // 4
+    hi = min(max_bytes, 1 << 30)
+    target = int((lo * hi) ** 0.5)
+    cov = 50.0
+    g = max(1000, int(target / (cov * 1.1)))
+    spec = gen.ReadSetSpec(genome_len=g, read_len=100, coverage=cov, err=0.005,
+                           genome_seed=9001 + g % 997, read_seed=9002 + g % 991)
+    return gen.make_fasta(spec)
+
+
+def load(path: str = DEFAULT_PATH) -> list[dict]:
+    if not os.path.exists(path):
+        return []
+    with open(path) as f:
+        return json.load(f)["entries"]
+
+
+def for_size(entries: list[dict], nbytes: int) -> dict:
+    """The tuning entry for an input of nbytes ({} = built-in choices)."""
+    for e in sorted(entries, key=lambda e: e["max_bytes"]):
+        if nbytes <= e["max_bytes"]:
+            return {x: e[x] for x in ("l1_chunks", "l1_bits", "bucket_factor")}
+    return {}
+
+
+def tune(path: str = DEFAULT_PATH, reps: int = 5, k: int = 32, t: int = 21) -> dict:
+    import numpy as np
+    import torch
+    from pakman_plus_amd import api
+    ctx = api.Context(0)
+    entries = []
+    prev = 0
+    for mb in BUCKETS:
+        fa = _sample(mb, prev)
+        dev = torch.from_numpy(fa).to("cuda:0")
+        torch.cuda.synchronize()
+        timings = {}
+        for cand in CANDIDATES:
+            ctx.set_tuning(**cand)
+            ts = []
+            for r in range(reps + 1):
+                t0 = time.perf_counter()
+                ctx.count(None, k, device_ptr=dev.data_ptr(), nbytes=dev.numel())
+                try:
+                    ctx.cutoff(t)
+                    ctx.macro_nodes()
+                except api.PakHipError as e:
+                    if e.code != api.PK_EEMPTY:
+                        raise
+                if r:
+                    ts.append(time.perf_counter() - t0)
+            timings[json.dumps(cand, sort_keys=True)] = float(np.median(ts))
+        best = min(timings, key=timings.get)
+        e = dict(json.loads(best), max_bytes=mb, sample_bytes=int(fa.nbytes), ms=timings[best] * 1e3,
+                 builtin_ms=None)
+        ctx.set_tuning()
+        ts = []
+        for r in range(reps + 1):
+            t0 = time.perf_counter()
+            ctx.count(None, k, device_ptr=dev.data_ptr(), nbytes=dev.numel())
+            try:
+                ctx.cutoff(t)
+                ctx.macro_nodes()
+            except api.PakHipError:
+                pass
+            if r:
+                ts.append(time.perf_counter() - t0)
+        e["builtin_ms"] = float(np.median(ts)) * 1e3
+        entries.append(e)
+        print(json.dumps(e), flush=True)
+        prev = mb
+        del dev
+    ctx.close()
+    out = {"device": torch.cuda.get_device_name(0), "k": k, "t": t, "reps": reps, "entries": entries}
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    return out
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=DEFAULT_PATH)
+    ap.add_argument("--reps", type=int, default=5)
+    a = ap.parse_args()
+    tune(a.out, a.reps)

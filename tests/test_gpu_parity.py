@@ -244,3 +244,23 @@ def test_rank_share(ctx, oracle_mod, k, P):
             # k-mer (the 3-bit code of the rolling 8-mer), so one k-mer value
             # can reach two ranks -- the oracle agrees rank by rank
             assert total >= ctx.table.n
+
+
+@pytest.mark.parametrize("tuning", [dict(l1_chunks=1, l1_bits=8, bucket_factor=1.5),
+                                    dict(l1_chunks=3, l1_bits=6, bucket_factor=5.0),
+                                    dict(l1_chunks=8, l1_bits=4, bucket_factor=0.5)])
+def test_tuning_does_not_change_results(ctx, oracle_mod, tuning):
+    """Launch-shape choices of the autotuner leave every output unchanged."""
+    _p(f"case: tuning {tuning}")
+    fa = read_fasta("syn_mid.fa.gz")
+    try:
+        ctx.set_tuning(**tuning)
+        for k in (32, 64):
+            keys, counts = _check_table(ctx, oracle_mod, fa, k)
+            hist, mb, nk = ctx.cutoff(21)
+            ohist, omb, okeep, _ = oracle_mod.cutoff(counts, 21)
+            assert np.array_equal(hist, ohist) and mb == omb and nk == int(okeep.sum())
+            kk, kc = ctx.kept_host()
+            _check_nodes(ctx, oracle_mod, k, kk, kc)
+    finally:
+        ctx.set_tuning()
