@@ -170,7 +170,14 @@ int pk_builder_count(pk_ctx* ctx, const char* fasta, uint64_t nbytes, int klen, 
  * the reference's default 21 (distribute_kmers.cpp:291-294).  Returns
  * PK_EEMPTY when nothing is kept. */
 int pk_cutoff(pk_ctx* ctx, int t, uint64_t* hist_t, int* min_bucket, uint64_t* n_kept);
-/* The kept table of the last pk_cutoff. */
+/* Threshold cutoff of the PaK-graph builder (Pakman_Subgraph::build_graph,
+ * src/pakman_graph.hpp:592-610: k-mers with count <= threshold are skipped;
+ * default_pakgraph_builder.cpp:39-48 passes t - 1): the kept table = entries
+ * with count > threshold, in table order.  No histogram; an empty kept table
+ * is not an error.  pk_macro_nodes then builds that builder's nodes (the
+ * same (k-1)-mer records; visit counts come from visit_value). */
+int pk_cutoff_threshold(pk_ctx* ctx, uint64_t threshold, uint64_t* n_kept);
+/* The kept table of the last pk_cutoff / pk_cutoff_threshold. */
 int pk_kept_table(pk_ctx* ctx, pk_table* out);
 
 /* Macro nodes of the kept table (mn_construct.cpp:212-477, one rank). */

@@ -16,6 +16,9 @@
 //   pakhip::visit_value / pakhip::reference_prefix_order
 //       the per-edge visit count and the reference's prefix order, for
 //       callers that fill the reference's MacroNode records.
+//   pakhip::build_pakgraph_nodes
+//       the node records of the PaK-graph builder (Pakman_Subgraph::
+//       build_graph, src/pakman_graph.hpp:592-673) at one rank.
 //
 // Header-only; link with -lpakhip.  Host code only (no HIP headers needed).
 #ifndef PAKHIP_BUILDERS_HPP
@@ -258,6 +261,105 @@ inline std::vector<std::array<uint8_t, 4>> reference_prefix_order(const Kmer_Tab
       }
   }
   return order;
+}
+
+// -- PaK-graph builder nodes ---------------------------------------------------
+//
+// Pakman_Subgraph::build_graph (src/pakman_graph.hpp:592-673) at one rank:
+// k-mers with count > threshold, grouped by (k-1)-mer; every node holds a
+// terminal prefix and suffix ("" , 0, 0) plus one single-base entry per kept
+// neighbour (count, visit = ceil(count / coverage)); node_t::
+// rebuild_prefix_suffix_connections (:357-386) then sorts both lists with
+// visit_count_sorter_t (:80-108), gives the terminal of the lighter side the
+// difference of the visit totals (:389-408) and wires prefixes to suffixes.
+// The device supplies the kept table and the (k-1)-mer records
+// (pk_cutoff_threshold + pk_macro_nodes); this assembles the node records.
+struct Pakgraph_Entry {
+  char base;  // 'A' 'C' 'T' 'G', or 0 for the terminal ("")
+  int count;
+  int visit;
+  bool terminal;
+  struct Wire {
+    size_t suffix_idx;
+    int count;
+    int offset;
+  };
+  std::vector<Wire> wires;  // prefixes only
+};
+
+struct Pakgraph_Node {
+  uint64_t key_lo, key_hi;  // (k-1)-mer
+  std::vector<Pakgraph_Entry> prefixes, suffixes;
+};
+
+inline bool pakgraph_entry_before(const Pakgraph_Entry& a, const Pakgraph_Entry& b) {
+  if (a.visit != b.visit) return a.visit > b.visit;
+  if (a.count != b.count) return a.count > b.count;
+  if (a.terminal != b.terminal) return !a.terminal > !b.terminal;
+  // suffix_ordering: shorter string first, then symbol value A < C < T < G
+  const int la = a.base ? 1 : 0, lb = b.base ? 1 : 0;
+  if (la != lb) return la < lb;
+  auto val = [](char c) { return c == 'A' ? 0 : c == 'C' ? 1 : c == 'T' ? 2 : 3; };
+  return la ? val(a.base) < val(b.base) : false;
+}
+
+inline void pakgraph_rebuild(Pakgraph_Node& n) {
+  std::sort(n.prefixes.begin(), n.prefixes.end(), pakgraph_entry_before);
+  std::sort(n.suffixes.begin(), n.suffixes.end(), pakgraph_entry_before);
+  int tp = 0, ts = 0;
+  for (const auto& p : n.prefixes) tp += p.visit;
+  for (const auto& s : n.suffixes) ts += s.visit;
+  if (ts < tp) {
+    n.suffixes.back().visit = tp - ts;
+    n.suffixes.back().count = 1;
+  } else if (ts > tp) {
+    n.prefixes.back().visit = ts - tp;
+    n.prefixes.back().count = 1;
+  }
+  size_t pi = 0, si = 0;
+  int pu = 0, su = 0;
+  while (pi < n.prefixes.size() && si < n.suffixes.size()) {
+    const int delta = std::min(n.prefixes[pi].visit - pu, n.suffixes[si].visit - su);
+    pu += delta;
+    su += delta;
+    n.prefixes[pi].wires.push_back({si, delta, su});
+    if (pu == n.prefixes[pi].visit) {
+      ++pi;
+      pu = 0;
+    }
+    if (su == n.suffixes[si].visit) {
+      ++si;
+      su = 0;
+    }
+  }
+}
+
+// Nodes of the PaK-graph builder over the current table of ctx (after a
+// count): threshold as Pakman_Subgraph::build_graph takes it (the harness
+// passes t - 1), coverage as its `cov` argument.
+inline std::vector<Pakgraph_Node> build_pakgraph_nodes(pk_ctx* ctx, uint64_t threshold, int coverage) {
+  uint64_t nk = 0;
+  throw_on_error(ctx, pk_cutoff_threshold(ctx, threshold, &nk), "pk_cutoff_threshold");
+  std::vector<Pakgraph_Node> out;
+  if (nk == 0) return out;
+  const std::vector<pk_mnode> recs = macro_nodes(ctx);
+  static const char kBase[4] = {'A', 'C', 'T', 'G'};
+  out.resize(recs.size());
+  for (size_t i = 0; i < recs.size(); i++) {
+    Pakgraph_Node& n = out[i];
+    n.key_lo = recs[i].key_lo;
+    n.key_hi = recs[i].key_hi;
+    n.prefixes.push_back({0, 0, 0, true, {}});
+    n.suffixes.push_back({0, 0, 0, true, {}});
+    for (int b = 0; b < 4; b++) {
+      if (recs[i].pre_cnt[b])
+        n.prefixes.push_back({kBase[b], recs[i].pre_cnt[b], visit_value(recs[i].pre_cnt[b], coverage), false, {}});
+      if (recs[i].suf_cnt[b])
+        n.suffixes.push_back({kBase[b], recs[i].suf_cnt[b], visit_value(recs[i].suf_cnt[b], coverage), false, {}});
+    }
+    pakgraph_rebuild(n);
+  }
+  return out;
 }
 
 }  // namespace pakhip

@@ -83,7 +83,7 @@ EXPORTS = ("pk_ctx_create", "pk_ctx_destroy", "pk_last_error", "pk_ctx_stream",
            "pk_ctx_device_bytes", "pk_ctx_set_tuning", "pk_ctx_release_buffers", "pk_count",
            "pk_count_rank_share",
            "pk_lmer_frequency", "pk_builder_count",
-           "pk_cutoff", "pk_kept_table", "pk_macro_nodes", "pk_download_table", "pk_download_kept",
+           "pk_cutoff", "pk_cutoff_threshold", "pk_kept_table", "pk_macro_nodes", "pk_download_table", "pk_download_kept",
            "pk_download_nodes", "pk_last_timing", "pk_server_create", "pk_server_destroy",
            "pk_server_now", "pk_server_submit", "pk_server_submit_batch", "pk_server_wait",
            "pk_server_poll", "pk_server_download", "pk_server_set_tuning", "pk_server_release",
@@ -129,6 +129,8 @@ def load_library():
     L.pk_builder_count.restype = i32
     L.pk_cutoff.argtypes = [vp, i32, vp, ctypes.POINTER(i32), ctypes.POINTER(u64)]
     L.pk_cutoff.restype = i32
+    L.pk_cutoff_threshold.argtypes = [vp, u64, ctypes.POINTER(u64)]
+    L.pk_cutoff_threshold.restype = i32
     L.pk_kept_table.argtypes = [vp, ctypes.POINTER(_Table)]
     L.pk_kept_table.restype = i32
     L.pk_macro_nodes.argtypes = [vp, ctypes.POINTER(_MnTable)]
@@ -318,6 +320,14 @@ class Context:
         self.hist = hist
         self._check(rc)
         return hist, int(mb.value), int(nk.value)
+
+    def cutoff_threshold(self, threshold: int) -> int:
+        """Keep count > threshold (Pakman_Subgraph::build_graph,
+        pakman_graph.hpp:592-610); returns the kept count."""
+        nk = ctypes.c_uint64()
+        self._check(self._lib.pk_cutoff_threshold(self._h, threshold, ctypes.byref(nk)))
+        self.n_kept = int(nk.value)
+        return self.n_kept
 
     def kept_host(self):
         w = self.table.words

@@ -2537,8 +2537,12 @@ int builder_count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int klen, uint6
                                          ~Key(0));
 }
 
+__global__ void k_set_int(int* p, int v) { *p = v; }
+
+// fixed_mb > 0: keep count >= fixed_mb with no histogram (the PaK-graph
+// builder's threshold rule); else the histogram cutoff with t buckets.
 template <class Key>
-int cutoff_impl(pk_ctx* c, int t, uint64_t* hist_out, int* mb_out, uint64_t* nk_out) {
+int cutoff_impl(pk_ctx* c, int t, uint64_t* hist_out, int* mb_out, uint64_t* nk_out, int fixed_mb = 0) {
   hipStream_t st = c->stream;
   PK_HIP(c, hipEventRecord(c->ev[12], st));
   Key* TK = reinterpret_cast<Key*>(c->buf[B_TK].p);
@@ -2554,7 +2558,14 @@ int cutoff_impl(pk_ctx* c, int t, uint64_t* hist_out, int* mb_out, uint64_t* nk_
   ull *bs, *boffs;
   PK_TRY(need(c, B_KBS, nb + 1, &bs));
   PK_TRY(need(c, B_KBOFF, nb + 1, &boffs));
-  if (t <= CUT_SMALL_T) {
+  if (fixed_mb > 0) {
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, mbd, fixed_mb);
+    PK_HIP(c, hipGetLastError());
+    if (nb) {
+      hipLaunchKernelGGL(k_kept_count, dim3((unsigned)nb), dim3(CUT_NT), 0, st, TC, U, mbd, bs);
+      PK_HIP(c, hipGetLastError());
+    }
+  } else if (t <= CUT_SMALL_T) {
     uint32_t* BH;
     PK_TRY(need(c, B_CUTBH, nb * (uint64_t)t + 1, &BH));
     if (nb) {
@@ -2587,7 +2598,8 @@ int cutoff_impl(pk_ctx* c, int t, uint64_t* hist_out, int* mb_out, uint64_t* nk_
   int mb = 0;
   PK_HIP(c, hipMemcpyAsync(&V, boffs + nb, 8, hipMemcpyDeviceToHost, st));
   PK_HIP(c, hipMemcpyAsync(&mb, mbd, 4, hipMemcpyDeviceToHost, st));
-  if (hist_out) PK_HIP(c, hipMemcpyAsync(hist_out, hist, (size_t)t * 8, hipMemcpyDeviceToHost, st));
+  if (hist_out && fixed_mb <= 0)
+    PK_HIP(c, hipMemcpyAsync(hist_out, hist, (size_t)t * 8, hipMemcpyDeviceToHost, st));
   PK_HIP(c, hipStreamSynchronize(st));
   Key* KK;
   int32_t* KC;
@@ -2610,7 +2622,7 @@ int cutoff_impl(pk_ctx* c, int t, uint64_t* hist_out, int* mb_out, uint64_t* nk_
   c->have_nodes = false;
   *mb_out = mb;
   *nk_out = V;
-  if (V == 0)
+  if (V == 0 && fixed_mb <= 0)
     return c->fail(PK_EEMPTY, "no k-mer has count >= min_bucket(" + std::to_string(mb) + ")");
   return PK_OK;
 }
@@ -2845,6 +2857,17 @@ int pk_cutoff(pk_ctx* c, int t, uint64_t* hist_t, int* min_bucket, uint64_t* n_k
   PK_HIP(c, hipSetDevice(c->device));
   return c->words == 1 ? cutoff_impl<uint64_t>(c, t, hist_t, min_bucket, n_kept)
                        : cutoff_impl<u128>(c, t, hist_t, min_bucket, n_kept);
+}
+
+int pk_cutoff_threshold(pk_ctx* c, uint64_t threshold, uint64_t* n_kept) {
+  if (!c || !n_kept) return PK_EINVAL;
+  if (!c->have_table) return c->fail(PK_ESTATE, "pk_cutoff_threshold before a count");
+  PK_HIP(c, hipSetDevice(c->device));
+  // counts are int32: a threshold at or above INT32_MAX keeps nothing
+  const int mb = threshold >= 0x7FFFFFFFull ? 0x7FFFFFFF : (int)threshold + 1;
+  int mbo = 0;
+  return c->words == 1 ? cutoff_impl<uint64_t>(c, 1, nullptr, &mbo, n_kept, mb)
+                       : cutoff_impl<u128>(c, 1, nullptr, &mbo, n_kept, mb);
 }
 
 int pk_kept_table(pk_ctx* c, pk_table* out) {

@@ -46,8 +46,37 @@ int run(const std::string& input, size_t klen, size_t read_len, size_t max_kmer_
   return 0;
 }
 
+// pakgraph FILE K THRESHOLD COV: main-path count of FILE, then the PaK-graph
+// builder nodes; one line per node:
+//   key_lo key_hi | base:count:visit:term[:sidx/count/offset...] ... | suffixes
+static int run_pakgraph(const char* path, int k, unsigned long threshold, int cov) {
+  FILE* f = fopen(path, "rb");
+  if (!f) throw std::logic_error(std::string("Unable to open file: ") + path);
+  std::string input;
+  char buf[1 << 16];
+  size_t r;
+  while ((r = fread(buf, 1, sizeof buf, f)) > 0) input.append(buf, r);
+  fclose(f);
+  pakhip::Device_Context ctx(0);
+  pakhip::count_kmers(ctx.get(), input.data(), input.size(), k);
+  auto nodes = pakhip::build_pakgraph_nodes(ctx.get(), threshold, cov);
+  for (const auto& n : nodes) {
+    printf("%lu %lu |", (unsigned long)n.key_lo, (unsigned long)n.key_hi);
+    for (const auto& e : n.prefixes) {
+      printf(" %c:%d:%d:%d", e.base ? e.base : '-', e.count, e.visit, (int)e.terminal);
+      for (const auto& w : e.wires) printf(":%zu/%d/%d", w.suffix_idx, w.count, w.offset);
+    }
+    printf(" |");
+    for (const auto& e : n.suffixes) printf(" %c:%d:%d:%d", e.base ? e.base : '-', e.count, e.visit, (int)e.terminal);
+    printf("\n");
+  }
+  return 0;
+}
+
 int main(int argc, char** argv) {
   try {
+    if (argc == 6 && std::string(argv[1]) == "pakgraph")
+      return run_pakgraph(argv[2], atoi(argv[3]), strtoul(argv[4], nullptr, 10), atoi(argv[5]));
     if (argc < 2) {
       // src/concurrent_kmer_counter_test.cpp:121-126
       const std::string input =

@@ -158,3 +158,34 @@ def test_cpp_adapter_reference_shape(oracle_mod, tmp_path):
     oreads, ok, oc = oracle_mod.builder_count(fa, 32, 100, 64)
     assert r["calls"] == -(-ok.shape[0] // 1000) and r["unique"] == ok.shape[0]
     assert r["xor"] == _digest(ok, oc)
+
+
+@pytest.mark.parametrize("name,k,threshold,cov", [("syn_mid.fa.gz", 32, 20, 100), ("syn_rep.fa.gz", 32, 14, 30),
+                                                  ("syn_mid.fa.gz", 64, 10, 50)])
+def test_pakgraph_nodes(oracle_mod, tmp_path, name, k, threshold, cov):
+    """The PaK-graph builder's nodes (count > threshold, sorted entries,
+    balanced terminals, wires) from the device records through the C++
+    adapter, against the restatement in tests/pakgraph_util.py."""
+    _p(f"case: pakgraph {name} k={k} threshold={threshold}")
+    from pakgraph_util import pakgraph_lines
+    fa = read_fasta(name)
+    f = tmp_path / "in.fa"
+    f.write_bytes(fa)
+    out = subprocess.run([ADAPTER, "pakgraph", str(f), str(k), str(threshold), str(cov)],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr
+    got = out.stdout.strip().splitlines()
+    want = pakgraph_lines(oracle_mod, fa, k, threshold, cov)
+    assert len(got) == len(want) > 0
+    assert got == want
+
+
+def test_cutoff_threshold(ctx, oracle_mod):
+    _p("case: threshold cutoff")
+    fa = read_fasta("syn_mid.fa.gz")
+    keys, counts = _check(ctx, oracle_mod, fa, 32, 150, 64)[1:]
+    for thr in (0, 1, 20, 10 ** 6):
+        nk = ctx.cutoff_threshold(thr)
+        kk, kc = ctx.kept_host()
+        sel = counts > thr  # pakman_graph.hpp:603 skips count <= threshold
+        assert nk == int(sel.sum()) and np.array_equal(kk, keys[sel]) and np.array_equal(kc, counts[sel])
