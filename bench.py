@@ -98,7 +98,7 @@ def run_plain_job(ctx, dev, k, t, table=None):
     With a tuning table the job's size entry is applied first (as the
     server does per job)."""
     from pakman_plus_amd import api, tune
-    ctx.set_tuning(**tune.for_size(table or [], int(dev.numel())))
+    ctx.set_tuning(**tune.for_size(table or [], int(dev.numel()), k))
     t0 = time.perf_counter()
     ctx.count(None, k, device_ptr=dev.data_ptr(), nbytes=dev.numel())
     U = ctx.table.n
@@ -281,7 +281,7 @@ def main():
     ctx = api.Context(0)
     from pakman_plus_amd import tune
     tuning_table = tune.load()
-    tun = tune.for_size(tuning_table, nbytes)
+    tun = tune.for_size(tuning_table, nbytes, k)
     if tun:
         ctx.set_tuning(**tun)
 

@@ -245,11 +245,13 @@ int pk_server_poll(pk_server* srv, uint64_t ticket, pk_job_result* out);
 int pk_server_download(pk_server* srv, uint64_t ticket, uint64_t* keys, int32_t* counts,
                        uint64_t* kept_keys, int32_t* kept_counts, uint64_t* hist_t,
                        pk_mnode* nodes);
-/* Per-size tuning for the jobs of a server: a job of nbytes uses the first
- * entry (ascending max_bytes) with nbytes <= max_bytes, else the built-in
+/* Per-size tuning for the jobs of a server: a job of nbytes with key words
+ * w (1 for k <= 32, else 2) uses the first entry (ascending max_bytes) with
+ * nbytes <= max_bytes and key_words == w (or 0 = any), else the built-in
  * choices.  Copies the table; n == 0 clears it. */
 typedef struct pk_tune_entry {
   uint64_t max_bytes;
+  int32_t key_words;
   pk_tuning tuning;
 } pk_tune_entry;
 int pk_server_set_tuning(pk_server* srv, const pk_tune_entry* entries, int n);

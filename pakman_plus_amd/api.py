@@ -57,7 +57,7 @@ class Tuning(ctypes.Structure):
 
 
 class _TuneEntry(ctypes.Structure):
-    _fields_ = [("max_bytes", ctypes.c_uint64), ("tuning", Tuning)]
+    _fields_ = [("max_bytes", ctypes.c_uint64), ("key_words", ctypes.c_int32), ("tuning", Tuning)]
 
 
 class _Job(ctypes.Structure):
@@ -497,12 +497,13 @@ class Server:
         self._t.pop(ticket, None)
 
     def set_tuning(self, entries: list[dict]):
-        """entries: dicts with max_bytes, l1_chunks, l1_bits, bucket_factor
-        (the autotuner's table, pakman_plus_amd.tune)."""
+        """entries: dicts with max_bytes, key_words, l1_chunks, l1_bits,
+        bucket_factor (the autotuner's table, pakman_plus_amd.tune)."""
         arr = (_TuneEntry * max(len(entries), 1))()
         for i, e in enumerate(entries):
-            arr[i] = _TuneEntry(int(e["max_bytes"]), Tuning(int(e.get("l1_chunks", 0)), int(e.get("l1_bits", 0)),
-                                                             float(e.get("bucket_factor", 0.0))))
+            arr[i] = _TuneEntry(int(e["max_bytes"]), int(e.get("key_words", 0)),
+                                Tuning(int(e.get("l1_chunks", 0)), int(e.get("l1_bits", 0)),
+                                       float(e.get("bucket_factor", 0.0))))
         self._check(self._lib.pk_server_set_tuning(self._h, arr, len(entries)))
 
 

@@ -1138,9 +1138,10 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
       if (threadIdx.x == 0) over = 0;
       __syncthreads();
       // keys are loaded HB at a time into registers so that the global-load
-      // latency is paid once per batch, not once per insert
+      // latency is paid once per batch, not once per insert; a task of one
+      // batch keeps them there for the fingerprint check below
+      Key kb[HB];
       for (uint64_t b = 0; b < n; b += (uint64_t)BS_NT * HB) {
-        Key kb[HB];
 #pragma unroll
         for (int j = 0; j < HB; j++) {
           const uint64_t idx = b + threadIdx.x + (uint64_t)j * BS_NT;
@@ -1157,12 +1158,14 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
       __syncthreads();
       if constexpr (sizeof(Key) == 16) {
         if (!over) {
+          const bool one_batch = n <= (uint64_t)BS_NT * HB;
           for (uint64_t b = 0; b < n; b += (uint64_t)BS_NT * HB) {
-            Key kb[HB];
+            if (!one_batch) {
 #pragma unroll
-            for (int j = 0; j < HB; j++) {
-              const uint64_t idx = b + threadIdx.x + (uint64_t)j * BS_NT;
-              kb[j] = idx < n ? S[start + idx] : Key(0);
+              for (int j = 0; j < HB; j++) {
+                const uint64_t idx = b + threadIdx.x + (uint64_t)j * BS_NT;
+                kb[j] = idx < n ? S[start + idx] : Key(0);
+              }
             }
 #pragma unroll
             for (int j = 0; j < HB; j++) {

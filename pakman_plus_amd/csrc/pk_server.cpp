@@ -101,8 +101,9 @@ void pk_server::run(int slot, Job* j) {
   pk_tuning tu{};
   {
     std::lock_guard<std::mutex> lk(mu);
+    const int words = j->spec.k <= 32 ? 1 : 2;
     for (const auto& e : tune)
-      if (j->spec.nbytes <= e.max_bytes) {
+      if (j->spec.nbytes <= e.max_bytes && (e.key_words == 0 || e.key_words == words)) {
         tu = e.tuning;
         break;
       }

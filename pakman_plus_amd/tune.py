@@ -1,8 +1,9 @@
 """Autotuner of the counting kernels' launch shapes, per input-size bucket.
 
-For every size bucket (FASTA bytes up to max_bytes) it builds a read set of
-that size shaped like the batched workload (uniform genome, 100 bp reads,
-0.5% errors), times count + cutoff + macro nodes on the device for each
+For every size bucket (FASTA bytes up to max_bytes) and key width (k = 32:
+one word; k = 64: two) it builds a read set of that size (uniform genome,
+100 or 150 bp reads, 50x, 0.5% errors), times count + cutoff + macro nodes
+on the device for each
 candidate (reads per level-1 block, level-1 digit bits, bucket factor = the
 inverse of the hash-table load target) and keeps the fastest.  The table is
 written as JSON (pakman_plus_amd/tuning.json) and applied by Server.set_tuning
@@ -25,15 +26,15 @@ CANDIDATES = ([{"l1_chunks": c, "l1_bits": 8, "bucket_factor": f}
               + [{"l1_chunks": 0, "l1_bits": b, "bucket_factor": 3.0} for b in (6, 7)])
 
 
-def _sample(max_bytes: int, prev: int):
+def _sample(max_bytes: int, prev: int, read_len: int):
     """A read set near the geometric middle of the bucket (prev, max_bytes]."""
     from pakman_plus_amd import gen
     lo = prev if prev else max_bytes // 4
     hi = min(max_bytes, 1 << 30)
     target = int((lo * hi) ** 0.5)
     cov = 50.0
-    g = max(1000, int(target / (cov * 1.1)))
-    spec = gen.ReadSetSpec(genome_len=g, read_len=100, coverage=cov, err=0.005,
+    g = max(1000, int(target / (cov * (1 + 12 / read_len))))
+    spec = gen.ReadSetSpec(genome_len=g, read_len=read_len, coverage=cov, err=0.005,
                            genome_seed=9001 + g % 997, read_seed=9002 + g % 991)
     return gen.make_fasta(spec)
 
@@ -45,15 +46,30 @@ def load(path: str = DEFAULT_PATH) -> list[dict]:
         return json.load(f)["entries"]
 
 
-def for_size(entries: list[dict], nbytes: int) -> dict:
-    """The tuning entry for an input of nbytes ({} = built-in choices)."""
+def for_size(entries: list[dict], nbytes: int, k: int = 32) -> dict:
+    """The tuning entry for an input of nbytes at k ({} = built-in choices)."""
+    words = 1 if k <= 32 else 2
     for e in sorted(entries, key=lambda e: e["max_bytes"]):
-        if nbytes <= e["max_bytes"]:
+        if nbytes <= e["max_bytes"] and e.get("key_words", 0) in (0, words):
             return {x: e[x] for x in ("l1_chunks", "l1_bits", "bucket_factor")}
     return {}
 
 
-def tune(path: str = DEFAULT_PATH, reps: int = 5, k: int = 32, t: int = 21) -> dict:
+def tune(path: str = DEFAULT_PATH, reps: int = 5, ks=(32, 64), t: int = 21) -> dict:
+    """Tunes every size bucket for one-word keys (k = 32) and two-word keys
+    (k = 64): the hash tables of the two hold different numbers of keys."""
+    entries = []
+    for k in ks:
+        entries += _tune_k(reps, k, t)
+    import torch
+    out = {"device": torch.cuda.get_device_name(0), "ks": list(ks), "t": t, "reps": reps,
+           "entries": entries}
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    return out
+
+
+def _tune_k(reps: int, k: int, t: int) -> list[dict]:
     import numpy as np
     import torch
     from pakman_plus_amd import api
@@ -61,7 +77,7 @@ def tune(path: str = DEFAULT_PATH, reps: int = 5, k: int = 32, t: int = 21) -> d
     entries = []
     prev = 0
     for mb in BUCKETS:
-        fa = _sample(mb, prev)
+        fa = _sample(mb, prev, 100 if k <= 32 else 150)
         dev = torch.from_numpy(fa).to("cuda:0")
         torch.cuda.synchronize()
         timings = {}
@@ -81,8 +97,8 @@ def tune(path: str = DEFAULT_PATH, reps: int = 5, k: int = 32, t: int = 21) -> d
                     ts.append(time.perf_counter() - t0)
             timings[json.dumps(cand, sort_keys=True)] = float(np.median(ts))
         best = min(timings, key=timings.get)
-        e = dict(json.loads(best), max_bytes=mb, sample_bytes=int(fa.nbytes), ms=timings[best] * 1e3,
-                 builtin_ms=None)
+        e = dict(json.loads(best), max_bytes=mb, key_words=1 if k <= 32 else 2, k=k,
+                 sample_bytes=int(fa.nbytes), ms=timings[best] * 1e3, builtin_ms=None)
         ctx.set_tuning()
         ts = []
         for r in range(reps + 1):
@@ -101,10 +117,7 @@ def tune(path: str = DEFAULT_PATH, reps: int = 5, k: int = 32, t: int = 21) -> d
         prev = mb
         del dev
     ctx.close()
-    out = {"device": torch.cuda.get_device_name(0), "k": k, "t": t, "reps": reps, "entries": entries}
-    with open(path, "w") as f:
-        json.dump(out, f, indent=1)
-    return out
+    return entries
 
 
 if __name__ == "__main__":
