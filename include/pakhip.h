@@ -20,6 +20,12 @@
  *
  * Threading: a context is used by one host thread at a time; contexts are
  * independent (each owns a HIP stream and its device memory).
+ *
+ * Ordering: the last kernels of a call may still run on the context's
+ * stream when it returns (the sizes it reports are final).  Later calls on
+ * the context, pk_download_* and pk_last_timing are ordered after them; a
+ * caller that reads the device pointers by other means orders itself after
+ * pk_ctx_stream() first.
  */
 #ifndef PAKHIP_H
 #define PAKHIP_H

@@ -1894,6 +1894,8 @@ struct pk_ctx {
   bool have_table = false, have_kept = false, have_nodes = false;
   int t = 0, mb = 0;
   int key_bits = 0;  // bits of a table key: 2k (main path) or the word (builder)
+  // stages whose event times have not been read yet
+  bool pend_count = false, pend_split = false, pend_cutoff = false, pend_mn = false;
   pk_tuning tuning{};  // zeros: built-in choices
 
   int fail(int code, const std::string& what) {
@@ -2396,8 +2398,14 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
                      Res, C, boff, bcnt, B, KB / 2, btot, tu, small + 4, ovl);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[7], st));
-  ull novf = 0;
-  PK_TRY(d2h(c, &novf, small + 4, 8));
+  // the scan of the distinct counts goes ahead of the overflow check: with
+  // no oversize bucket (the usual case) one synchronisation yields both
+  PK_TRY(scan_excl<long long>(c, tu, uoff, B));
+  ull hv[2];
+  PK_HIP(c, hipMemcpyAsync(&hv[0], small + 4, 8, hipMemcpyDeviceToHost, st));
+  PK_HIP(c, hipMemcpyAsync(&hv[1], uoff + B, 8, hipMemcpyDeviceToHost, st));
+  PK_HIP(c, hipStreamSynchronize(st));
+  const ull novf = hv[0];
   c->timing.n_split = (int32_t)novf;
   PK_HIP(c, hipEventRecord(c->ev[8], st));
   const ull* lstart = boff;
@@ -2447,9 +2455,11 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
     lu = lud;
   }
   PK_HIP(c, hipEventRecord(c->ev[9], st));
-  PK_TRY(scan_excl<long long>(c, lu, uoff, nleaves));
-  ull U = 0;
-  PK_TRY(d2h(c, &U, uoff + nleaves, 8));
+  ull U = hv[1];
+  if (novf) {
+    PK_TRY(scan_excl<long long>(c, lu, uoff, nleaves));
+    PK_TRY(d2h(c, &U, uoff + nleaves, 8));
+  }
   c->U = U;
   PK_TRY(need(c, B_TK, U, &TK));
   PK_TRY(need(c, B_TC, U, &TC));
@@ -2458,19 +2468,10 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
                      C, lstart, lu, uoff, nleaves, TK, TC);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[11], st));
-  PK_HIP(c, hipStreamSynchronize(st));
-  auto el = [&](int a, int b) {
-    float v = 0;
-    (void)hipEventElapsedTime(&v, c->ev[a], c->ev[b]);
-    return v;
-  };
-  c->timing.parse_ms = el(0, 1);
-  c->timing.hist_ms = el(2, 3);
-  c->timing.scatter_ms = el(4, 5);
-  c->timing.l1_scatter_ms = el(4, 16);
-  c->timing.sort_ms = el(6, 7);
-  c->timing.split_ms = novf ? el(8, 9) : 0.f;
-  c->timing.compact_ms = el(10, 11);
+  // stage times are read from the events when asked for (pk_last_timing):
+  // no wait here, the compaction runs on while the caller goes on
+  c->pend_count = true;
+  c->pend_split = novf != 0;
   c->have_table = true;
   return PK_OK;
 }
@@ -2479,6 +2480,7 @@ template <class Key>
 int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
   c->have_table = c->have_kept = c->have_nodes = false;
   c->timing = pk_timing{};
+  c->pend_count = c->pend_split = c->pend_cutoff = c->pend_mn = false;
   c->key_bits = 2 * K;
   PK_HIP(c, hipEventRecord(c->ev[0], c->stream));
   ull* rstart;
@@ -2513,6 +2515,7 @@ template <class Key>
 int rank_share_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K, int P, int rank) {
   c->have_table = c->have_kept = c->have_nodes = false;
   c->timing = pk_timing{};
+  c->pend_count = c->pend_split = c->pend_cutoff = c->pend_mn = false;
   c->key_bits = 2 * K;
   PK_HIP(c, hipEventRecord(c->ev[0], c->stream));
   ull* freq;
@@ -2530,6 +2533,7 @@ int builder_count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int klen, uint6
                        BldSyms sy) {
   c->have_table = c->have_kept = c->have_nodes = false;
   c->timing = pk_timing{};
+  c->pend_count = c->pend_split = c->pend_cutoff = c->pend_mn = false;
   c->key_bits = (int)sizeof(Key) * 8;
   PK_HIP(c, hipEventRecord(c->ev[0], c->stream));
   ull* rstart;
@@ -2614,10 +2618,7 @@ int cutoff_impl(pk_ctx* c, int t, uint64_t* hist_out, int* mb_out, uint64_t* nk_
     PK_HIP(c, hipGetLastError());
   }
   PK_HIP(c, hipEventRecord(c->ev[13], st));
-  PK_HIP(c, hipStreamSynchronize(st));
-  float ms = 0;
-  PK_HIP(c, hipEventElapsedTime(&ms, c->ev[12], c->ev[13]));
-  c->timing.cutoff_ms = ms;
+  c->pend_cutoff = true;
   c->V = V;
   c->t = t;
   c->mb = mb;
@@ -2670,10 +2671,7 @@ int mn_impl(pk_ctx* c) {
   hipLaunchKernelGGL(k_mn_compact, dim3((unsigned)G), dim3(256), 0, st, NT, plan, ng, noff, G, nodes);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[15], st));
-  PK_HIP(c, hipStreamSynchronize(st));
-  float ms = 0;
-  PK_HIP(c, hipEventElapsedTime(&ms, c->ev[14], c->ev[15]));
-  c->timing.mn_ms = ms;
+  c->pend_mn = true;
   c->M = M;
   c->have_nodes = true;
   return PK_OK;
@@ -2924,8 +2922,36 @@ int pk_download_nodes(pk_ctx* c, pk_mnode* nodes) {
   return d2h(c, nodes, c->buf[B_NODES].p, c->M * sizeof(pk_mnode));
 }
 
-int pk_last_timing(const pk_ctx* c, pk_timing* out) {
-  if (!c || !out) return PK_EINVAL;
+int pk_last_timing(const pk_ctx* cc, pk_timing* out) {
+  if (!cc || !out) return PK_EINVAL;
+  pk_ctx* c = const_cast<pk_ctx*>(cc);
+  auto el = [&](int a, int b) {
+    float v = 0;
+    (void)hipEventElapsedTime(&v, c->ev[a], c->ev[b]);
+    return v;
+  };
+  if (c->pend_count || c->pend_cutoff || c->pend_mn) {
+    PK_HIP(c, hipSetDevice(c->device));
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  if (c->pend_count) {
+    c->timing.parse_ms = el(0, 1);
+    c->timing.hist_ms = el(2, 3);
+    c->timing.scatter_ms = el(4, 5);
+    c->timing.l1_scatter_ms = el(4, 16);
+    c->timing.sort_ms = el(6, 7);
+    c->timing.split_ms = c->pend_split ? el(8, 9) : 0.f;
+    c->timing.compact_ms = el(10, 11);
+    c->pend_count = false;
+  }
+  if (c->pend_cutoff) {
+    c->timing.cutoff_ms = el(12, 13);
+    c->pend_cutoff = false;
+  }
+  if (c->pend_mn) {
+    c->timing.mn_ms = el(14, 15);
+    c->pend_mn = false;
+  }
   *out = c->timing;
   return PK_OK;
 }
