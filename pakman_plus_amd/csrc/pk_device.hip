@@ -1387,12 +1387,26 @@ __global__ __launch_bounds__(CUT_NT) void k_count_hist_blocks(const int32_t* TC,
   for (int i = threadIdx.x; i < NW * t; i += CUT_NT) h[i] = 0;
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER + (uint64_t)threadIdx.x * 4;
+  int v[4];
+  if (base + 4 <= U) {
+    const int4 q = *reinterpret_cast<const int4*>(TC + base);
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+    v[3] = q.w;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; i++) v[i] = base + i < U ? TC[base + i] : -1;
+  }
+  // count 1 (the k-mers seen once, most of the table) is summed per wave by
+  // ballot instead of 64 lanes hitting one LDS word
+  const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    if (base + i < U) {
-      const int v = TC[base + i];
-      if (v >= 0 && v < t) atomicAdd(&h[w * t + v], 1u);
-    }
+    const bool one = v[i] == 1 && t > 1;
+    const unsigned long long m = __ballot(one);
+    if (lane == 0 && m) atomicAdd(&h[w * t + 1], (uint32_t)__popcll(m));
+    if (!one && v[i] >= 0 && v[i] < t) atomicAdd(&h[w * t + v[i]], 1u);
   }
   __syncthreads();
   for (int c = threadIdx.x; c < t; c += CUT_NT) {
