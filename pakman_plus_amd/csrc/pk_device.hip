@@ -7,26 +7,29 @@
 //   2. read table     k_reads: record r = header line j0+2r, sequence line
 //                     j0+2r+1, with the record rules and the format check of
 //                     the reference's Sliding_window (src/kmer_counting.cpp:
-//                     2890-2945); k-mer offsets by exclusive scan.
+//                     2890-2945); k-mer offsets by exclusive scan.  The
+//                     builder API's records come from k_bld_flags/k_bld_write.
 //   3. partition      multisplit by the top bits of the k-mer, <= 8 bits per
 //                     level.  Level 1 (k_l1_hist + k_l1_scatter) is fused with
-//                     the parse: every read is walked by one thread with the
-//                     reference's rolling 3-bit update (kmer_shift,
-//                     distribute_kmers.h:798-805); tiles of k-mers are sorted
-//                     by digit in LDS and written as contiguous runs.  Further
-//                     levels (k_ms_hist + k_ms_scatter) split each bucket.
-//                     This is the on-device stand-in for the reference's
-//                     owner routing + MPI_Alltoallv (kmer_counting.cpp:254-352).
-//   4. bucket count   k_bucket_hash: one 512-thread workgroup per bucket counts
+//                     the parse: each chunk of reads is staged in LDS and every
+//                     read is walked by one thread with the reference's rolling
+//                     3-bit update (kmer_shift, distribute_kmers.h:798-805);
+//                     tiles of k-mers are sorted by digit in LDS and written as
+//                     contiguous runs.  Further levels (k_ms_hist +
+//                     k_ms_scatter) split each bucket.  This is the on-device
+//                     stand-in for the reference's owner routing +
+//                     MPI_Alltoallv (kmer_counting.cpp:254-352); rank-share
+//                     mode keeps one owner's k-mers (own_step).
+//   4. bucket count   k_bucket_hash: one 256-thread workgroup per bucket counts
 //                     the bucket's distinct keys in an LDS hash table, then
-//                     counting-sorts them by the next 12 bits and writes the
+//                     counting-sorts them by the next bits and writes the
 //                     ascending (key, count) pairs (the reference's sort + RLE
 //                     + merge + aggregate, kmer_counting.cpp:3057-3086,
 //                     169-250).  Buckets with more distinct keys than a table
-//                     holds are split further by k_part_hist / k_part_scatter
-//                     (host-driven levels).
+//                     holds take extra passes, then the host-driven split
+//                     k_part_hist / k_part_scatter.
 //   5. compaction     k_compact: bucket results -> dense sorted table.
-// pk_cutoff: k_count_hist (LDS histogram of counts < t), k_min_bucket, ordered
+// pk_cutoff: per-block LDS histograms of counts < t, k_min_bucket, ordered
 //   stream compaction (kmer_histogram.cpp:84-140).
 // pk_macro_nodes: range-partitioned 5-way merge of the kept table seen as
 //   pred keys (ascending) and, per first base a, succ keys (ascending within
@@ -69,9 +72,6 @@ __device__ __forceinline__ uint32_t key_digit(Key x, int pos, int nbits) {
   }
   return (uint32_t)y & ((1u << nbits) - 1u);
 }
-
-template <class Key>
-__device__ __forceinline__ Key load_key(const Key* p) { return *p; }
 
 // ---------------------------------------------------------------------------
 // Block-wide primitives (wave64).
@@ -149,25 +149,6 @@ __device__ __forceinline__ ull block_sum(ull v) {
   ull s = 0;
 #pragma unroll
   for (int i = 0; i < NW; i++) s += ws[i];
-  __syncthreads();
-  return s;
-}
-
-template <int NT>
-__device__ __forceinline__ ull block_max(ull v) {
-  constexpr int NW = NT / 64;
-  __shared__ ull ws[NW];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    ull y = __shfl_xor(v, o, 64);
-    v = y > v ? y : v;
-  }
-  if (lane == 0) ws[w] = v;
-  __syncthreads();
-  ull s = 0;
-#pragma unroll
-  for (int i = 0; i < NW; i++) s = ws[i] > s ? ws[i] : s;
   __syncthreads();
   return s;
 }
@@ -435,35 +416,6 @@ __device__ __forceinline__ uint32_t base_code(uint32_t c) {
   return acgt ? (c >> 1) & 3u : 0u;
 }
 
-// Calls f(kmer) for every k-mer of bytes [s, s+len), in read order:
-// kmer = ((kmer << 2) | code(c)) & mask.
-template <class Key, int ENC, class F>
-__device__ __forceinline__ void for_each_kmer(const uint8_t* fa, uint64_t n, uint64_t s,
-                                              uint32_t len, int K, Key mask, F&& f) {
-  if (len < (uint32_t)K) return;
-  const uint64_t e = s + len;
-  const bool aligned = (reinterpret_cast<uintptr_t>(fa) & 7) == 0;
-  Key km = 0;
-  uint64_t wa = s & ~7ull;
-  uint64_t w = aligned ? load_word8(fa, n, wa) : 0;
-  uint32_t i = 0;
-  for (uint64_t p = s; p < e; ++p, ++i) {
-    uint32_t c;
-    if (aligned) {
-      const uint64_t a = p & ~7ull;
-      if (a != wa) {
-        wa = a;
-        w = load_word8(fa, n, wa);
-      }
-      c = (uint32_t)(w >> ((p & 7) * 8)) & 0xFFu;
-    } else {
-      c = fa[p];
-    }
-    km = ((km << 2) | (Key)base_code<ENC>(c)) & mask;
-    if (i + 1 >= (uint32_t)K) f(km);
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Partition (multisplit) into buckets by the top bits of the key, at most 8
 // bits per level so that every tile holds tens of keys per digit: each tile
@@ -478,6 +430,20 @@ template <class Key>
 struct TileCfg {
   static constexpr int Q = sizeof(Key) == 8 ? 16 : 8;  // keys per thread per tile
   static constexpr int TQ = L1_NT * Q;                 // 4096 (u64) / 2048 (u128): 32 KB
+};
+
+// LDS of a tile split by digit: a histogram and its exclusive scan, a
+// counting-sort scatter of the keys into T1 (digit of every sorted slot in
+// D1), then coalesced stores: sorted slot p of digit d goes to
+// out[base[d] - toff[d] + p] (delta[d] = base[d] - toff[d]).
+template <class Key, int NT>
+struct TileLds {
+  Key T1[NT * TileCfg<Key>::Q];
+  uint8_t D1[NT * TileCfg<Key>::Q];
+  uint32_t h[256];
+  uint32_t tcur[256];
+  ull base[256];
+  long long delta[256];
 };
 
 // Level-1 producer.  A block owns rpb consecutive reads and walks them in
@@ -518,11 +484,6 @@ __device__ __forceinline__ void stage_chunk(ChunkLds& C, const uint8_t* fa, uint
     for (ull i = threadIdx.x; i < nw; i += NT) C.sb[i] = load_word8(fa, n, first + 8 * i);
   }
   __syncthreads();
-}
-
-__device__ __forceinline__ uint32_t chunk_byte(const ChunkLds& C, const uint8_t* fa, ull p) {
-  if (C.lds) return reinterpret_cast<const uint8_t*>(C.sb)[p - C.base];
-  return fa[p];
 }
 
 // Walks up to Q more k-mers of the thread's read (state s, i, km): emits
@@ -677,49 +638,6 @@ __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n
     __syncthreads();
   }
   for (int i = threadIdx.x; i < nb; i += L1_NT) H[(ull)i * gridDim.x + blockIdx.x] = h[i];
-}
-
-// Split a tile by digit: LDS histogram and exclusive scan (toff), a
-// counting-sort scatter of the keys into T1 (digit of every sorted slot in
-// D1), then coalesced stores: sorted slot p of digit d goes to
-// out[base[d] - toff[d] + p].  Keys come from get(j), j < np.
-template <class Key, int NT>
-struct TileLds {
-  Key T1[NT * TileCfg<Key>::Q];
-  uint8_t D1[NT * TileCfg<Key>::Q];
-  uint32_t h[256];
-  uint32_t tcur[256];
-  ull base[256];
-  long long delta[256];
-};
-
-template <class Key, int NT, class Get>
-__device__ __forceinline__ void tile_split_write(TileLds<Key, NT>& L, int np, Get&& get, int dpos,
-                                                 int dbits, Key* out) {
-  const int nb = 1 << dbits;
-  for (int i = threadIdx.x; i < nb; i += NT) L.h[i] = 0;
-  __syncthreads();
-  for (int j = 0; j < np; j++) atomicAdd(&L.h[key_digit<Key>(get(j), dpos, dbits)], 1u);
-  __syncthreads();
-  // nb <= 256 == NT: one digit per thread
-  const uint32_t v = (int)threadIdx.x < nb ? L.h[threadIdx.x] : 0u;
-  const ull ex = block_excl_scan<NT>(v);
-  if ((int)threadIdx.x < nb) {
-    L.tcur[threadIdx.x] = (uint32_t)ex;
-    L.delta[threadIdx.x] = (long long)L.base[threadIdx.x] - (long long)ex;
-  }
-  const int total = (int)block_sum<NT>(v);
-  for (int j = 0; j < np; j++) {
-    const Key k = get(j);
-    const uint32_t d = key_digit<Key>(k, dpos, dbits);
-    const uint32_t pos = atomicAdd(&L.tcur[d], 1u);
-    L.T1[pos] = k;
-    L.D1[pos] = (uint8_t)d;
-  }
-  __syncthreads();
-  for (int p = threadIdx.x; p < total; p += NT) out[L.delta[L.D1[p]] + p] = L.T1[p];
-  if ((int)threadIdx.x < nb) L.base[threadIdx.x] += v;
-  __syncthreads();
 }
 
 // Level-1 scatter fused with the parse: per chunk, every thread produces up

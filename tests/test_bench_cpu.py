@@ -1,0 +1,41 @@
+"""Host logic of bench.py and the tuning table (CPU only)."""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_algorithmic_bytes_formula():
+    sys.path.insert(0, ROOT)
+    import bench
+    # bacterial workload figures: S, N, U, V, M at k = 32 (W = 8)
+    S, N, U, V, M, W = 407368890, 253920000, 77049109, 9199758, 9199796, 8
+    ab = bench.alg_bytes(N, U, V, M, S, W)
+    want = (S + 2 * N * W + U * (W + 4) + U * 4 + U * (W + 4) + V * (W + 4) + 2 * V * (W + 4)
+            + M * (W + 32))
+    assert ab["pipeline"] == want
+    assert ab["l1_scatter"] == S + N * W and ab["sort"] == N * W + U * (W + 4)
+    assert 7.0e9 < want < 7.6e9
+
+
+def test_tuning_lookup():
+    sys.path.insert(0, ROOT)
+    from pakman_plus_amd import tune
+    entries = [{"max_bytes": 100, "key_words": 1, "l1_chunks": 1, "l1_bits": 8, "bucket_factor": 3.0},
+               {"max_bytes": 1000, "key_words": 1, "l1_chunks": 2, "l1_bits": 7, "bucket_factor": 5.0},
+               {"max_bytes": 1000, "key_words": 2, "l1_chunks": 4, "l1_bits": 8, "bucket_factor": 1.5}]
+    assert tune.for_size(entries, 50, 32)["l1_chunks"] == 1
+    assert tune.for_size(entries, 500, 32)["l1_bits"] == 7
+    assert tune.for_size(entries, 500, 64)["bucket_factor"] == 1.5
+    assert tune.for_size(entries, 5000, 32) == {}
+    table = tune.load()
+    assert table and {e["key_words"] for e in table} == {1, 2}
