@@ -189,6 +189,34 @@ int pk_kept_table(pk_ctx* ctx, pk_table* out);
 /* Macro nodes of the kept table (mn_construct.cpp:212-477, one rank). */
 int pk_macro_nodes(pk_ctx* ctx, pk_mn_table* out);
 
+/* Initial wiring of the macro nodes (initiate_mnode_wiring, src/wire.cpp:
+ * 66-121, and MacroNode::setup_wiring, includes/distribute_kmers.h:550-656):
+ * per node the prefix list (pre_order, or ascending base when pre_order is
+ * NULL) and the suffix list (ascending base) of single-base entries with
+ * (count, visit = ceil(count / coverage)), each followed by a terminal; the
+ * terminal of the lighter side gets the difference of the visit totals; then
+ * prefixes and suffixes in descending (visit, count) order are wired.
+ * pre_order: host array of 4 bases per node (the prefixes' order, 255 =
+ * unused), e.g. from pakhip::reference_prefix_order to match a reference
+ * run.  out: host array of one record per node of the last pk_macro_nodes. */
+typedef struct pk_wire_info {
+  int32_t suffix_id;
+  int32_t offset_in_suffix;
+  int32_t count;
+} pk_wire_info;
+
+typedef struct pk_mnwire {
+  int32_t n_prefix;  /* including the terminal */
+  int32_t n_suffix;  /* including the terminal */
+  int32_t n_wire;    /* filled entries of wire */
+  int32_t pre_terminal_count[2];  /* (count, visit) of the prefix terminal */
+  int32_t suf_terminal_count[2];  /* (count, visit) of the suffix terminal */
+  pk_wire_info wire[11];          /* n_prefix + n_suffix + 1 entries in the reference */
+  int32_t prefix_begin[5][2];     /* per prefix: (first wire, number of wires) */
+} pk_mnwire;
+
+int pk_mn_wiring(pk_ctx* ctx, int coverage, const uint8_t* pre_order, pk_mnwire* out);
+
 /* Copy the current table / kept table / macro nodes to host memory.
  * Arrays must hold n*words keys, n counts, n nodes. */
 int pk_download_table(pk_ctx* ctx, uint64_t* keys, int32_t* counts);

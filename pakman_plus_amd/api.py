@@ -75,6 +75,10 @@ class _JobResult(ctypes.Structure):
                 ("message", ctypes.c_char * 160)]
 
 
+MNWIRE_DTYPE = np.dtype([("n_prefix", "<i4"), ("n_suffix", "<i4"), ("n_wire", "<i4"),
+                         ("pre_terminal", "<i4", (2,)), ("suf_terminal", "<i4", (2,)),
+                         ("wire", "<i4", (11, 3)), ("prefix_begin", "<i4", (5, 2))])
+
 MNODE_DTYPE = np.dtype([("key_lo", "<u8"), ("key_hi", "<u8"), ("pre_cnt", "<i4", (4,)),
                         ("suf_cnt", "<i4", (4,))])
 
@@ -83,7 +87,7 @@ EXPORTS = ("pk_ctx_create", "pk_ctx_destroy", "pk_last_error", "pk_ctx_stream",
            "pk_ctx_device_bytes", "pk_ctx_set_tuning", "pk_ctx_release_buffers", "pk_count",
            "pk_count_rank_share",
            "pk_lmer_frequency", "pk_builder_count",
-           "pk_cutoff", "pk_cutoff_threshold", "pk_kept_table", "pk_macro_nodes", "pk_download_table", "pk_download_kept",
+           "pk_cutoff", "pk_cutoff_threshold", "pk_kept_table", "pk_macro_nodes", "pk_mn_wiring", "pk_download_table", "pk_download_kept",
            "pk_download_nodes", "pk_last_timing", "pk_server_create", "pk_server_destroy",
            "pk_server_now", "pk_server_submit", "pk_server_submit_batch", "pk_server_wait",
            "pk_server_poll", "pk_server_download", "pk_server_set_tuning", "pk_server_release",
@@ -135,6 +139,8 @@ def load_library():
     L.pk_kept_table.restype = i32
     L.pk_macro_nodes.argtypes = [vp, ctypes.POINTER(_MnTable)]
     L.pk_macro_nodes.restype = i32
+    L.pk_mn_wiring.argtypes = [vp, i32, vp, vp]
+    L.pk_mn_wiring.restype = i32
     L.pk_download_table.argtypes = [vp, vp, vp]
     L.pk_download_table.restype = i32
     L.pk_download_kept.argtypes = [vp, vp, vp]
@@ -343,6 +349,19 @@ class Context:
         self.n_nodes = int(m.n)
         self.nodes_ptr = m.nodes or 0
         return self.n_nodes
+
+    def mn_wiring(self, coverage: int, pre_order: np.ndarray | None = None) -> np.ndarray:
+        """Initial wiring of the last macro nodes (wire.cpp:66-121,
+        distribute_kmers.h:550-656); pre_order: uint8 [n_nodes, 4] prefix
+        bases in order (255 = unused), None = ascending base."""
+        out = np.zeros(self.n_nodes, dtype=MNWIRE_DTYPE)
+        po = None
+        if pre_order is not None:
+            po = np.ascontiguousarray(pre_order, dtype=np.uint8)
+            assert po.shape == (self.n_nodes, 4)
+        self._check(self._lib.pk_mn_wiring(self._h, coverage, po.ctypes.data if po is not None else None,
+                                           out.ctypes.data))
+        return out
 
     def nodes_host(self) -> np.ndarray:
         out = np.empty(self.n_nodes, dtype=MNODE_DTYPE)

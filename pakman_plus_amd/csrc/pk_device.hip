@@ -1783,6 +1783,114 @@ __global__ __launch_bounds__(MN_NT) void k_mn_build(const Key* KK, const int32_t
   if (threadIdx.x == 0) ng[g] = nodes;
 }
 
+// Initial wiring of one macro node per thread (MacroNode::setup_wiring,
+// includes/distribute_kmers.h:550-656, after initiate_mnode_wiring appends
+// the terminals, src/wire.cpp:84-96).  Entry lists: prefixes in the given
+// base order, suffixes ascending, each followed by its terminal; counts are
+// (count, visit) pairs.  The index sorts (Comp_rev: visit, then count,
+// descending) run on at most five entries, where libstdc++'s std::sort is an
+// insertion sort, i.e. stable -- so is the insertion sort here.
+__device__ __forceinline__ void wire_sort(int* idx, int n, const int (*cnt)[2]) {
+  for (int i = 1; i < n; i++) {
+    const int v = idx[i];
+    int j = i;
+    while (j > 0) {
+      const int u = idx[j - 1];
+      const bool before = cnt[v][1] > cnt[u][1] || (cnt[v][1] == cnt[u][1] && cnt[v][0] > cnt[u][0]);
+      if (!before) break;
+      idx[j] = u;
+      j--;
+    }
+    idx[j] = v;
+  }
+}
+
+__global__ void k_mn_wire(const MnRec* nodes, uint64_t M, const uint8_t* pre_order, int cov,
+                          pk_mnwire* out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  const MnRec r = nodes[i];
+  int pc_[6][2], sc_[6][2];
+  int np = 0, ns = 0;
+  for (int q = 0; q < 4; q++) {
+    int b = q;
+    if (pre_order) {
+      const uint8_t o = pre_order[i * 4 + q];
+      if (o > 3) continue;
+      b = o;
+    }
+    if (r.pre[b]) {
+      pc_[np][0] = r.pre[b];
+      pc_[np][1] = (r.pre[b] + cov - 1) / cov;
+      np++;
+    }
+  }
+  for (int b = 0; b < 4; b++)
+    if (r.suf[b]) {
+      sc_[ns][0] = r.suf[b];
+      sc_[ns][1] = (r.suf[b] + cov - 1) / cov;
+      ns++;
+    }
+  int pc = 0, sc = 0;
+  for (int a = 0; a < np; a++) pc += pc_[a][1];
+  for (int a = 0; a < ns; a++) sc += sc_[a][1];
+  // terminals (count -1 in the reference until setup_wiring balances them)
+  pc_[np][0] = 1;
+  pc_[np][1] = sc - pc > 0 ? sc - pc : 0;
+  sc_[ns][0] = 1;
+  sc_[ns][1] = pc - sc > 0 ? pc - sc : 0;
+  np++;
+  ns++;
+  int ip[6], is[6];
+  for (int a = 0; a < np; a++) ip[a] = a;
+  for (int a = 0; a < ns; a++) is[a] = a;
+  wire_sort(ip, np, pc_);
+  wire_sort(is, ns, sc_);
+  pk_mnwire w;
+  w.n_prefix = np;
+  w.n_suffix = ns;
+  w.pre_terminal_count[0] = pc_[np - 1][0];
+  w.pre_terminal_count[1] = pc_[np - 1][1];
+  w.suf_terminal_count[0] = sc_[ns - 1][0];
+  w.suf_terminal_count[1] = sc_[ns - 1][1];
+  for (int a = 0; a < 11; a++) w.wire[a] = pk_wire_info{0, 0, 0};
+  for (int a = 0; a < 5; a++) w.prefix_begin[a][0] = w.prefix_begin[a][1] = 0;
+  int off[6] = {0, 0, 0, 0, 0, 0};
+  int leftover = sc + sc_[ns - 1][1];
+  int top_p = 0, top_s = 0, var_p = 0, var_s = 0, wi = 0, p_size = 0, last_pid = -1, begin = -1;
+  while (leftover > 0 && wi < 11 && top_p < np && top_s < ns) {
+    const int sid = is[top_s], pid = ip[top_p];
+    const int a = pc_[pid][1] - var_p, b = sc_[sid][1] - var_s;
+    const int c = a < b ? a : b;
+    w.wire[wi] = pk_wire_info{sid, off[sid], c};
+    if (last_pid != pid) {
+      begin = wi;
+      last_pid = pid;
+    }
+    p_size++;
+    wi++;
+    leftover -= c;
+    var_p += c;
+    var_s += c;
+    off[sid] += c;
+    if (var_p == pc_[pid][1]) {
+      var_p = 0;
+      top_p++;
+      if (pid < 5) {
+        w.prefix_begin[pid][0] = begin;
+        w.prefix_begin[pid][1] = p_size;
+      }
+      p_size = 0;
+    }
+    if (var_s == sc_[sid][1]) {
+      var_s = 0;
+      top_s++;
+    }
+  }
+  w.n_wire = wi;
+  out[i] = w;
+}
+
 __global__ void k_mn_compact(const MnRec* NT, const MnPlan* plan, const ull* ng, const ull* noff,
                              uint64_t G, MnRec* out) {
   const uint64_t g = blockIdx.x;
@@ -1805,7 +1913,7 @@ enum BufId {
   B_FASTA, B_BLK, B_BLKOFF, B_NL, B_RSTART, B_RLEN, B_KC, B_KOFS, B_SMALL, B_BCNT, B_BOFF, B_BCUR,
   B_S, B_C, B_TU, B_UOFF, B_TK, B_TC, B_HIST, B_KBS, B_KBOFF, B_KK, B_KC2, B_MNQ, B_MNPLAN,
   B_MNNT, B_MNNG, B_MNNOFF, B_MNFLAG, B_NODES, B_TMP, B_OVF, B_PT0, B_PT1, B_PT2, B_PT3, B_PT4,
-  B_PT5, B_PT6, B_BLKGT, B_S2, B_BCNT2, B_BOFF2, B_BCUR2, B_TP, B_HIST1, B_OFF1, B_HIST2, B_OFF2, B_CUTBH, B_LFREQ, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
+  B_PT5, B_PT6, B_BLKGT, B_S2, B_BCNT2, B_BOFF2, B_BCUR2, B_TP, B_HIST1, B_OFF1, B_HIST2, B_OFF2, B_CUTBH, B_LFREQ, B_WORDER, B_WIRE, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
 };
 
 }  // namespace
@@ -2827,6 +2935,24 @@ int pk_macro_nodes(pk_ctx* c, pk_mn_table* out) {
   out->k = c->k;
   out->nodes = reinterpret_cast<const pk_mnode*>(c->buf[B_NODES].p);
   return PK_OK;
+}
+
+int pk_mn_wiring(pk_ctx* c, int coverage, const uint8_t* pre_order, pk_mnwire* out) {
+  if (!c || !out || coverage < 1) return PK_EINVAL;
+  if (!c->have_nodes) return c->fail(PK_ESTATE, "pk_mn_wiring before pk_macro_nodes");
+  if (c->M == 0) return PK_OK;
+  PK_HIP(c, hipSetDevice(c->device));
+  uint8_t* ord = nullptr;
+  if (pre_order) {
+    PK_TRY(need(c, B_WORDER, c->M * 4, &ord));
+    PK_HIP(c, hipMemcpyAsync(ord, pre_order, c->M * 4, hipMemcpyHostToDevice, c->stream));
+  }
+  pk_mnwire* w;
+  PK_TRY(need(c, B_WIRE, c->M, &w));
+  hipLaunchKernelGGL(k_mn_wire, dim3(grid_for(c->M, 256)), dim3(256), 0, c->stream,
+                     reinterpret_cast<const MnRec*>(c->buf[B_NODES].p), c->M, ord, coverage, w);
+  PK_HIP(c, hipGetLastError());
+  return d2h(c, out, w, c->M * sizeof(pk_mnwire));
 }
 
 int pk_download_table(pk_ctx* c, uint64_t* keys, int32_t* counts) {

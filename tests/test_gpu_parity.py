@@ -264,3 +264,36 @@ def test_tuning_does_not_change_results(ctx, oracle_mod, tuning):
             _check_nodes(ctx, oracle_mod, k, kk, kc)
     finally:
         ctx.set_tuning()
+
+
+@pytest.mark.parametrize("name,k,cov,t", [(n, k, c, t) for n, (c, ks, t) in SYNTH.items() for k in ks])
+def test_initial_wiring_matches_reference(ctx, oracle_mod, name, k, cov, t):
+    """pk_mn_wiring with the reference's prefix order reproduces the wiring the
+    reference's own initiate_mnode_wiring / setup_wiring wrote (node dumps in
+    tests/golden): entry counts, terminal (count, visit), every wire and every
+    prefix's (first wire, number of wires)."""
+    _p(f"case: initial wiring {name} k={k}")
+    fa = read_fasta(f"syn_{name}.fa.gz")
+    ref = read_ref(f"syn_{name}_k{k}.ref.txt.gz")
+    ctx.count(fa, k)
+    ctx.cutoff(t)
+    kk, kc = ctx.kept_host()
+    m = ctx.macro_nodes()
+    nodes = ctx.nodes_host()
+    order = oracle_mod.macro_nodes(kk, kc, k, ref_order=True)["pre_order"]
+    w = ctx.mn_wiring(cov, order)
+    assert m == len(ref.nodes)
+    index = {(int(lo), int(hi)): i for i, (lo, hi) in enumerate(zip(nodes["key_lo"], nodes["key_hi"]))}
+    multi = 0
+    for node in ref.nodes:
+        i = index[node.key]
+        n_p, n_s = len(node.prefixes), len(node.suffixes)
+        assert (w["n_prefix"][i], w["n_suffix"][i]) == (n_p, n_s)
+        assert tuple(w["pre_terminal"][i]) == tuple(node.prefixes[-1][1:3])
+        assert tuple(w["suf_terminal"][i]) == tuple(node.suffixes[-1][1:3])
+        got = [tuple(int(x) for x in e) for e in w["wire"][i][:n_p + n_s + 1]]
+        assert got == node.wiring, (node.k1mer, got, node.wiring)
+        assert [tuple(int(x) for x in e) for e in w["prefix_begin"][i][:n_p]] == node.begin
+        multi += int(n_p > 2 or n_s > 2)
+    if name == "rep":
+        assert multi > 0  # repeat families leave branching nodes
