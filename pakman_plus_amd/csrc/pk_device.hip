@@ -9,14 +9,15 @@
 //                     the reference's Sliding_window (src/kmer_counting.cpp:
 //                     2890-2945); k-mer offsets by exclusive scan.  The
 //                     builder API's records come from k_bld_flags/k_bld_write.
-//   3. partition      multisplit by the top bits of the k-mer, <= 8 bits per
-//                     level.  Level 1 (k_l1_hist + k_l1_scatter) is fused with
-//                     the parse: each chunk of reads is staged in LDS and every
+//   3. partition      multisplit by the top bits of the k-mer.  Level 1
+//                     (k_l1_hist + k_l1_scatter, <= 10 bits) is fused with the
+//                     parse: each chunk of reads is staged in LDS and every
 //                     read is walked by one thread with the reference's rolling
 //                     3-bit update (kmer_shift, distribute_kmers.h:798-805);
-//                     tiles of k-mers are sorted by digit in LDS and written as
-//                     contiguous runs.  Further levels (k_ms_hist +
-//                     k_ms_scatter) split each bucket.  This is the on-device
+//                     every k-mer is stored at its block's next slot for its
+//                     digit.  Further levels (k_ms_hist + k_ms_scatter, <= 8
+//                     bits) split each bucket tile by tile, sorted by digit in
+//                     LDS and written as contiguous runs.  This is the on-device
 //                     stand-in for the reference's owner routing +
 //                     MPI_Alltoallv (kmer_counting.cpp:254-352); rank-share
 //                     mode keeps one owner's k-mers (own_step).
@@ -417,19 +418,22 @@ __device__ __forceinline__ uint32_t base_code(uint32_t c) {
 }
 
 // ---------------------------------------------------------------------------
-// Partition (multisplit) into buckets by the top bits of the key, at most 8
-// bits per level so that every tile holds tens of keys per digit: each tile
-// is sorted by digit in LDS and written out as contiguous runs.
-//   level 1  k_l1_hist / k_l1_scatter: fused with the parse of the reads
-//   level 2+ k_ms_hist / k_ms_scatter: from the previous level's array
+// Partition (multisplit) into buckets by the top bits of the key.
+//   level 1  k_l1_hist / k_l1_scatter: fused with the parse of the reads, up
+//            to 10 bits; each key goes straight to its slot
+//   level 2+ k_ms_hist / k_ms_scatter: from the previous level's array, at
+//            most 8 bits so that every tile holds tens of keys per digit:
+//            each tile is sorted by digit in LDS and written as runs
 
 constexpr int L1_NT = 256;
 constexpr int MS_NT = 512;
+constexpr int L1_MAX_BITS = 10;     // level-1 digit width (direct scatter)
+constexpr int L1_DEFAULT_BITS = 8;  // wider digits keep more partial lines open
+constexpr int MS_MAX_BITS = 8;   // further levels (tile split into runs)
 
 template <class Key>
 struct TileCfg {
   static constexpr int Q = sizeof(Key) == 8 ? 16 : 8;  // keys per thread per tile
-  static constexpr int TQ = L1_NT * Q;                 // 4096 (u64) / 2048 (u128): 32 KB
 };
 
 // LDS of a tile split by digit: a histogram and its exclusive scan, a
@@ -617,7 +621,8 @@ __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n
                                                    const uint32_t* rlen, uint64_t R, int K, int KB,
                                                    Key mask, int d1, int rpb, OwnCfg cf, uint32_t* H) {
   __shared__ ChunkLds C;
-  __shared__ uint32_t h[256];
+  extern __shared__ uint32_t l1_dyn[];
+  uint32_t* h = l1_dyn;  // 2^d1 digit counts (dynamic LDS)
   const int nb = 1 << d1;
   for (int i = threadIdx.x; i < nb; i += L1_NT) h[i] = 0;
   const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
@@ -640,77 +645,46 @@ __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n
   for (int i = threadIdx.x; i < nb; i += L1_NT) H[(ull)i * gridDim.x + blockIdx.x] = h[i];
 }
 
-// Level-1 scatter fused with the parse: per chunk, every thread produces up
-// to Q keys of its read per tile (into its T1 slots, then registers); the
-// block splits the tile by digit in LDS and writes contiguous runs at its
-// digit bases (scanned histogram O, same layout as H).
-// qt (<= Q) keys per thread per tile: chosen so that a typical read's
-// k-mers fill whole tiles (69 k-mers: 5 tiles of 14 rather than 16+16+16+16+5).
+// Level-1 scatter fused with the parse: the same chunked walk as k_l1_hist;
+// every key takes the next slot of its digit from the block's LDS counter
+// (atomicAdd's return value) and is stored at once at the block's base for
+// that digit (scanned histogram O, same layout as H).  No tile buffer and no
+// barrier inside the walk: the order of keys inside a bucket is arbitrary,
+// which the bucket count does not depend on.
 template <class Key, int ENC, bool OWN>
 __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_t n,
                                                       const ull* rstart, const uint32_t* rlen,
                                                       uint64_t R, int K, int KB, Key mask, int d1,
-                                                      int rpb, int qt, OwnCfg cf, const ull* O,
-                                                      Key* S1) {
-  constexpr int Q = TileCfg<Key>::Q;
-  __shared__ TileLds<Key, L1_NT> L;
+                                                      int rpb, OwnCfg cf, const ull* O, Key* S1) {
   __shared__ ChunkLds C;
+  extern __shared__ ull l1_dyn64[];
+  ull* base = l1_dyn64;                                     // 2^d1 output cursors
+  uint32_t* h = reinterpret_cast<uint32_t*>(l1_dyn64 + (1 << d1));  // 2^d1 ranks
   const int nb = 1 << d1;
   for (int i = threadIdx.x; i < nb; i += L1_NT) {
-    L.base[i] = O[(ull)i * gridDim.x + blockIdx.x];
-    L.h[i] = 0;
+    base[i] = O[(ull)i * gridDim.x + blockIdx.x];
+    h[i] = 0;
   }
   const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
   const uint64_t r1 = r0 + rpb < R ? r0 + rpb : R;
   const int bpos = KB - d1;
+  // stage_chunk's barriers order the set-up above before the first walk
   for (uint64_t c0 = r0; c0 < r1;) {
     stage_chunk(C, fa, n, rstart, rlen, c0, r1, L1_NT);
     const int m = C.m;
-    const bool mine = (int)threadIdx.x < m;
-    const ull s = mine ? rstart[c0 + threadIdx.x] : 0;
-    const uint32_t len = mine ? rlen[c0 + threadIdx.x] : 0;
-    uint32_t i = 0;
-    Key km = 0;
-    OwnState os{0, 0, 0, 0};
-    // Barriers per tile: the loop test, histogram done, two in the scan,
-    // ranks ready, tile sorted.  Each thread resets its own histogram entry
-    // after reading it; the loop test orders that before the next tile's
-    // atomics, and the write-out's T1 reads before the next walk's stores.
-    while (__syncthreads_or(i < len)) {
-      Key* stage = L.T1 + threadIdx.x * Q;
-      const int np = walk_read<Key, ENC, OWN>(C, fa, s, len, i, km, os, cf, K, mask, qt,
-                                              [&](int q, Key x) { stage[q] = x; });
-      Key kr[Q];
-#pragma unroll
-      for (int q = 0; q < Q; q++) kr[q] = q < np ? stage[q] : Key(0);
-#pragma unroll
-      for (int q = 0; q < Q; q++)
-        if (q < np) atomicAdd(&L.h[key_digit<Key>(kr[q], bpos, d1)], 1u);
-      __syncthreads();
-      const uint32_t v = (int)threadIdx.x < nb ? L.h[threadIdx.x] : 0u;
-      ull tot;
-      const ull ex = block_scan_total<L1_NT>(v, &tot);
-      if ((int)threadIdx.x < nb) {
-        L.tcur[threadIdx.x] = (uint32_t)ex;
-        L.delta[threadIdx.x] = (long long)L.base[threadIdx.x] - (long long)ex;
-        L.base[threadIdx.x] += v;
-        L.h[threadIdx.x] = 0;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < Q; q++) {
-        if (q < np) {
-          const uint32_t d = key_digit<Key>(kr[q], bpos, d1);
-          const uint32_t pos = atomicAdd(&L.tcur[d], 1u);
-          L.T1[pos] = kr[q];
-          L.D1[pos] = (uint8_t)d;
-        }
-      }
-      __syncthreads();
-      const int total = (int)tot;
-      for (int p = threadIdx.x; p < total; p += L1_NT) S1[L.delta[L.D1[p]] + p] = L.T1[p];
+    if ((int)threadIdx.x < m) {
+      const uint64_t r = c0 + threadIdx.x;
+      uint32_t i = 0;
+      Key km = 0;
+      OwnState os{0, 0, 0, 0};
+      walk_read<Key, ENC, OWN>(C, fa, rstart[r], rlen[r], i, km, os, cf, K, mask, 0x7fffffff,
+                               [&](int, Key x) {
+                                 const uint32_t d = key_digit<Key>(x, bpos, d1);
+                                 S1[base[d] + atomicAdd(&h[d], 1u)] = x;
+                               });
     }
     c0 += m;
+    __syncthreads();
   }
 }
 
@@ -873,6 +847,10 @@ constexpr uint64_t TASK_MAX_INSTANCES = 1u << 18;
 // it sends the task to the next pass split (the host sizes buckets for a
 // load of about 0.75, where probes average a few slots)
 constexpr int HASH_PROBES = 256;
+// a probe chain longer than STOP_AFTER slots re-reads the task's overflow
+// flag every 8 probes: once one thread has found the table full, the others
+// stop probing it (short chains, the usual case, never read it)
+constexpr int STOP_AFTER = 8;
 
 __device__ __forceinline__ ull mix64(ull x) {
   x ^= x >> 33;
@@ -899,19 +877,25 @@ struct HashTable<uint64_t> {
       hc[i] = 0;
     }
   }
-  // returns false when no free slot is found within HASH_PROBES probes
-  __device__ __forceinline__ bool insert(uint64_t k) {
+  // returns false when no free slot is found within HASH_PROBES probes, or
+  // when a long chain finds the table already reported full (*stop)
+  __device__ __forceinline__ bool try_slot(uint32_t slot, uint64_t k) {
+    const ull prev = atomicCAS(&hk[slot], empty, (ull)k);
+    if (prev == empty || prev == (ull)k) {
+      atomicAdd(&hc[slot], 1u);
+      return true;
+    }
+    return false;
+  }
+  __device__ __forceinline__ bool insert(uint64_t k, const volatile int* stop) {
     uint32_t slot = (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> 40) & (T - 1);
-    for (int probe = 0; probe < HASH_PROBES; probe++) {
-      const ull prev = atomicCAS(&hk[slot], empty, (ull)k);
-      if (prev == empty) {
-        atomicAdd(&hc[slot], 1u);
-        return true;
-      }
-      if (prev == (ull)k) {
-        atomicAdd(&hc[slot], 1u);
-        return true;
-      }
+    for (int probe = 0; probe < STOP_AFTER; probe++) {
+      if (try_slot(slot, k)) return true;
+      slot = (slot + 1) & (T - 1);
+    }
+    for (int probe = STOP_AFTER; probe < HASH_PROBES; probe++) {
+      if ((probe & 7) == 0 && *stop) return false;
+      if (try_slot(slot, k)) return true;
       slot = (slot + 1) & (T - 1);
     }
     return false;
@@ -941,21 +925,26 @@ struct HashTable<u128> {
     }
   }
   __device__ __forceinline__ uint32_t home(ull f) const { return (uint32_t)(f >> 40) & (T - 1); }
-  __device__ __forceinline__ bool insert(u128 k) {
+  __device__ __forceinline__ bool try_slot(uint32_t slot, ull f, u128 k) {
+    const ull prev = atomicCAS(&hf[slot], EMPTY, f);
+    if (prev == EMPTY) hkf[slot] = k;
+    if (prev == EMPTY || prev == f) {
+      atomicAdd(&hc[slot], 1u);
+      return true;
+    }
+    return false;
+  }
+  __device__ __forceinline__ bool insert(u128 k, const volatile int* stop) {
     const ull f = fp(k);
     if (f == EMPTY) return false;
     uint32_t slot = home(f);
-    for (int probe = 0; probe < HASH_PROBES; probe++) {
-      const ull prev = atomicCAS(&hf[slot], EMPTY, f);
-      if (prev == EMPTY) {
-        hkf[slot] = k;
-        atomicAdd(&hc[slot], 1u);
-        return true;
-      }
-      if (prev == f) {
-        atomicAdd(&hc[slot], 1u);
-        return true;
-      }
+    for (int probe = 0; probe < STOP_AFTER; probe++) {
+      if (try_slot(slot, f, k)) return true;
+      slot = (slot + 1) & (T - 1);
+    }
+    for (int probe = STOP_AFTER; probe < HASH_PROBES; probe++) {
+      if ((probe & 7) == 0 && *stop) return false;
+      if (try_slot(slot, f, k)) return true;
       slot = (slot + 1) & (T - 1);
     }
     return false;
@@ -1069,7 +1058,7 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
 #pragma unroll
         for (int j = 0; j < HB; j++) {
           const uint64_t idx = b + threadIdx.x + (uint64_t)j * BS_NT;
-          if (idx < n && (int)key_digit<Key>(kb[j], rem - s, s) == pass && !ht.insert(kb[j]))
+          if (idx < n && (int)key_digit<Key>(kb[j], rem - s, s) == pass && !ht.insert(kb[j], &over))
             over = 1;
         }
       }
@@ -2322,7 +2311,7 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   const pk_tuning& tun = c->tuning;
   const float bf = tun.bucket_factor > 0.f ? tun.bucket_factor : 3.0f;
   const int btot = choose_bits(OWN ? (N + cf.P - 1) / cf.P : N, KB / 2, CAP, bf);
-  const int l1max = tun.l1_bits > 0 && tun.l1_bits < 8 ? tun.l1_bits : 8;
+  const int l1max = tun.l1_bits > 0 && tun.l1_bits <= L1_MAX_BITS ? tun.l1_bits : L1_DEFAULT_BITS;
   const int d1 = btot < l1max ? btot : l1max;
   const uint64_t B1 = 1ull << d1;
   // reads per block: up to 8 chunks of 256, fewer for small inputs so that
@@ -2331,13 +2320,6 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
                                ? tun.l1_chunks
                                : (int)std::min<uint64_t>(8, std::max<uint64_t>(1, R / (L1_NT * 1024ull))));
   const uint64_t nblk = (R + rpb - 1) / rpb;
-  int qt = TileCfg<Key>::Q;
-  {
-    const uint64_t kpr = (N + R - 1) / R;  // k-mers per read (mean)
-    const uint64_t tiles = (kpr + qt - 1) / qt;
-    if (tiles > 0) qt = (int)((kpr + tiles - 1) / tiles);
-    if (qt < 1) qt = 1;
-  }
   ull *boff, *O;
   uint32_t* H;
   Key* S2;
@@ -2349,7 +2331,8 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   PK_TRY(need(c, B_C, N, &C));
   c->timing.bucket_bits = btot;
   PK_HIP(c, hipEventRecord(c->ev[2], st));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_hist<Key, ENC, OWN>), dim3((unsigned)nblk), dim3(L1_NT), 0, st,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_hist<Key, ENC, OWN>), dim3((unsigned)nblk), dim3(L1_NT),
+                     (unsigned)(B1 * sizeof(uint32_t)), st,
                      fa, n, rstart, rlen, R, K, KB, mask, d1, rpb, cf, H);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[3], st));
@@ -2373,8 +2356,9 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
                      boff);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[4], st));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key, ENC, OWN>), dim3((unsigned)nblk), dim3(L1_NT), 0,
-                     st, fa, n, rstart, rlen, R, K, KB, mask, d1, rpb, qt, cf, O, S);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key, ENC, OWN>), dim3((unsigned)nblk), dim3(L1_NT),
+                     (unsigned)(B1 * (sizeof(ull) + sizeof(uint32_t))), st, fa, n, rstart, rlen, R, K, KB,
+                     mask, d1, rpb, cf, O, S);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[16], st));
   uint64_t B = B1;
@@ -2384,7 +2368,11 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   Key* Sout = S2;
   constexpr int TQ = MS_NT * TileCfg<Key>::Q;
   while (fixed < btot) {
-    const int db = btot - fixed < 8 ? btot - fixed : 8;
+    // the remaining bits over as few levels of <= MS_MAX_BITS as possible,
+    // evenly
+    const int left = btot - fixed;
+    const int nlev = (left + MS_MAX_BITS - 1) / MS_MAX_BITS;
+    const int db = (left + nlev - 1) / nlev;
     const uint64_t B2 = B << db;
     const uint64_t tiles = N / TQ + B + 1;  // upper bound on tp[B]
     ull *boff2, *tp, *O2;
@@ -2770,7 +2758,7 @@ int pk_ctx_set_tuning(pk_ctx* c, const pk_tuning* t) {
     c->tuning = pk_tuning{};
     return PK_OK;
   }
-  if (t->l1_chunks < 0 || t->l1_chunks > 64 || t->l1_bits < 0 || t->l1_bits > 8 ||
+  if (t->l1_chunks < 0 || t->l1_chunks > 64 || t->l1_bits < 0 || t->l1_bits > L1_MAX_BITS ||
       t->bucket_factor < 0.f || t->bucket_factor > 64.f)
     return c->fail(PK_EINVAL, "tuning values out of range");
   c->tuning = *t;

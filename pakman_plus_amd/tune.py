@@ -22,8 +22,10 @@ DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.
 # bucket upper bounds (FASTA bytes); the last one catches everything larger
 BUCKETS = [1 << 21, 1 << 23, 1 << 25, 1 << 27, 1 << 29, 1 << 62]
 CANDIDATES = ([{"l1_chunks": c, "l1_bits": 8, "bucket_factor": f}
-               for c in (1, 2, 4, 8) for f in (1.5, 3.0, 5.0)]
-              + [{"l1_chunks": 0, "l1_bits": b, "bucket_factor": 3.0} for b in (6, 7)])
+               for c in (1, 2, 4, 8) for f in (1.5, 2.2, 3.0, 5.0)]
+              + [{"l1_chunks": 0, "l1_bits": b, "bucket_factor": 3.0} for b in (6, 7)]
+              + [{"l1_chunks": 0, "l1_bits": b, "bucket_factor": f} for b in (9, 10)
+                 for f in (1.5, 2.2)])
 
 
 def _sample(max_bytes: int, prev: int, read_len: int):

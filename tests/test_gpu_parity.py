@@ -248,7 +248,11 @@ def test_rank_share(ctx, oracle_mod, k, P):
 
 @pytest.mark.parametrize("tuning", [dict(l1_chunks=1, l1_bits=8, bucket_factor=1.5),
                                     dict(l1_chunks=3, l1_bits=6, bucket_factor=5.0),
-                                    dict(l1_chunks=8, l1_bits=4, bucket_factor=0.5)])
+                                    dict(l1_chunks=8, l1_bits=4, bucket_factor=0.5),
+                                    dict(l1_chunks=2, l1_bits=10, bucket_factor=2.2),
+                                    # buckets far beyond a hash table: every
+                                    # bucket overflows its first passes
+                                    dict(l1_chunks=1, l1_bits=9, bucket_factor=24.0)])
 def test_tuning_does_not_change_results(ctx, oracle_mod, tuning):
     """Launch-shape choices of the autotuner leave every output unchanged."""
     _p(f"case: tuning {tuning}")
@@ -264,6 +268,15 @@ def test_tuning_does_not_change_results(ctx, oracle_mod, tuning):
             _check_nodes(ctx, oracle_mod, k, kk, kc)
     finally:
         ctx.set_tuning()
+
+
+def test_tuning_out_of_range(ctx):
+    from pakman_plus_amd import api
+    for bad in (dict(l1_bits=11), dict(l1_chunks=65), dict(bucket_factor=65.0)):
+        with pytest.raises(api.PakHipError) as e:
+            ctx.set_tuning(**bad)
+        assert e.value.code == api.PK_EINVAL
+    ctx.set_tuning()
 
 
 @pytest.mark.parametrize("name,k,cov,t", [(n, k, c, t) for n, (c, ks, t) in SYNTH.items() for k in ks])
