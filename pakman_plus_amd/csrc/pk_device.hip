@@ -233,33 +233,58 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_apply(const T* in, ull* out, c
 // Line index.
 
 constexpr int NL_NT = 256;
-constexpr int NL_PER_THREAD = 16;
-constexpr uint64_t NL_PER_BLOCK = (uint64_t)NL_NT * NL_PER_THREAD;
+// A block covers NL_SLABS slabs of NL_SLAB bytes; in slab j lane t reads the
+// 16 bytes at 16 t (one coalesced 16-byte load per lane and slab).
+constexpr int NL_SLABS = 4;
+constexpr uint64_t NL_SLAB = (uint64_t)NL_NT * 16;
+constexpr uint64_t NL_PER_BLOCK = NL_SLAB * NL_SLABS;
 
-__device__ __forceinline__ void load16(const uint8_t* fa, uint64_t n, uint64_t base, uint8_t* b) {
+// 4-bit mask of the bytes of w equal to e (bit i = byte i): a byte is zero
+// in t = w ^ eeee exactly when bit 7 of ((t & 7f..) + 7f..) | t is clear.
+__device__ __forceinline__ uint32_t match4(uint32_t w, uint32_t e) {
+  const uint32_t t = w ^ (e * 0x01010101u);
+  const uint32_t z = ~(((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t) & 0x80808080u;
+  return ((z >> 7) & 1u) | ((z >> 14) & 2u) | ((z >> 21) & 4u) | ((z >> 28) & 8u);
+}
+
+// Masks (bit i = byte base + i) of the event bytes (e0 or e1) and of the
+// '>' bytes among the 16 bytes at base (bytes at or past n never match).
+__device__ __forceinline__ void scan16(const uint8_t* fa, uint64_t n, uint64_t base, uint32_t e0,
+                                       uint32_t e1, uint32_t& ev, uint32_t& gt) {
+  ev = 0;
+  gt = 0;
   if (base + 16 <= n && ((reinterpret_cast<uintptr_t>(fa + base) & 15) == 0)) {
-    uint4 v = *reinterpret_cast<const uint4*>(fa + base);
-    memcpy(b, &v, 16);
-  } else {
+    const uint4 v = *reinterpret_cast<const uint4*>(fa + base);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-    for (int i = 0; i < 16; i++) b[i] = (base + i < n) ? fa[base + i] : 0;
+    for (int i = 0; i < 4; i++) {
+      ev |= (match4(w[i], e0) | match4(w[i], e1)) << (4 * i);
+      gt |= match4(w[i], '>') << (4 * i);
+    }
+  } else {
+    for (int i = 0; i < 16; i++) {
+      if (base + i >= n) break;
+      const uint32_t b = fa[base + i];
+      ev |= (uint32_t)(b == e0 || b == e1) << i;
+      gt |= (uint32_t)(b == '>') << i;
+    }
   }
 }
 
 // Event bytes of the index: bytes equal to e0 or e1 ('\n' and '\n' for the
 // line index of the main path; delimiter and skip symbol for the builder).
+// Per block: the number of event bytes and the first '>'.
 __global__ __launch_bounds__(NL_NT) void k_nl_count(const uint8_t* fa, uint64_t n, uint8_t e0,
                                                     uint8_t e1, ull* blk_cnt, ull* blk_gt) {
-  const uint64_t base = (uint64_t)blockIdx.x * NL_PER_BLOCK + (uint64_t)threadIdx.x * NL_PER_THREAD;
-  uint8_t b[16];
-  load16(fa, n, base, b);
+  const uint64_t b0 = (uint64_t)blockIdx.x * NL_PER_BLOCK + (uint64_t)threadIdx.x * 16;
+  uint32_t ev[NL_SLABS], gm[NL_SLABS];
+#pragma unroll
+  for (int j = 0; j < NL_SLABS; j++) scan16(fa, n, b0 + j * NL_SLAB, e0, e1, ev[j], gm[j]);
   ull c = 0, gt = ~0ull;
 #pragma unroll
-  for (int i = 0; i < 16; i++) {
-    if (base + i < n) {
-      c += (b[i] == e0 || b[i] == e1);
-      if (b[i] == '>' && gt == ~0ull) gt = base + i;
-    }
+  for (int j = NL_SLABS - 1; j >= 0; j--) {
+    c += __popc(ev[j]);
+    if (gm[j]) gt = b0 + j * NL_SLAB + __ffs(gm[j]) - 1;
   }
   c = block_sum<NL_NT>(c);
   gt = block_min<NL_NT>(gt);
@@ -277,18 +302,29 @@ __global__ __launch_bounds__(1024) void k_min_reduce(const ull* in, uint64_t n, 
   if (threadIdx.x == 0) *out = m;
 }
 
+// Positions of the event bytes, ascending, from the block's base blk_off.
+// The four slab counts of a thread travel through one block scan packed in
+// 16-bit fields (a slab holds at most 4096 events, so no field carries).
 __global__ __launch_bounds__(NL_NT) void k_nl_write(const uint8_t* fa, uint64_t n, uint8_t e0,
                                                     uint8_t e1, const ull* blk_off, ull* nl) {
-  const uint64_t base = (uint64_t)blockIdx.x * NL_PER_BLOCK + (uint64_t)threadIdx.x * NL_PER_THREAD;
-  uint8_t b[16];
-  load16(fa, n, base, b);
-  ull c = 0;
+  const uint64_t b0 = (uint64_t)blockIdx.x * NL_PER_BLOCK + (uint64_t)threadIdx.x * 16;
+  uint32_t ev[NL_SLABS], gm[NL_SLABS];
+  ull packed = 0;
 #pragma unroll
-  for (int i = 0; i < 16; i++) c += (base + i < n && (b[i] == e0 || b[i] == e1));
-  ull o = block_excl_scan<NL_NT>(c) + blk_off[blockIdx.x];
+  for (int j = 0; j < NL_SLABS; j++) {
+    scan16(fa, n, b0 + j * NL_SLAB, e0, e1, ev[j], gm[j]);
+    packed |= (ull)__popc(ev[j]) << (16 * j);
+  }
+  ull tot;
+  const ull ex = block_scan_total<NL_NT>(packed, &tot);
+  ull slab_base = blk_off[blockIdx.x];
 #pragma unroll
-  for (int i = 0; i < 16; i++)
-    if (base + i < n && (b[i] == e0 || b[i] == e1)) nl[o++] = base + i;
+  for (int j = 0; j < NL_SLABS; j++) {
+    ull o = slab_base + ((ex >> (16 * j)) & 0xFFFFull);
+    const uint64_t pos = b0 + j * NL_SLAB;
+    for (uint32_t m = ev[j]; m; m &= m - 1) nl[o++] = pos + __ffs(m) - 1;
+    slab_base += (tot >> (16 * j)) & 0xFFFFull;
+  }
 }
 
 // Builder-API records (Concurrent_Kmer_Partial_Counter::
