@@ -488,25 +488,29 @@ struct TileLds {
 
 // Level-1 producer.  A block owns rpb consecutive reads and walks them in
 // chunks: a chunk is the longest run of <= L1_NT consecutive reads whose
-// bytes fit in the LDS staging buffer, loaded with coalesced 8-byte loads;
-// thread t then walks read t of the chunk byte by byte from LDS with the
-// rolling update (a read too long for the buffer is walked from global
-// memory, alone in its chunk).  Records are in file order, so a chunk's
-// bytes are one contiguous range.
-constexpr int L1_SB = 32768;  // staging bytes
+// bytes fit in the LDS staging buffer; thread t then walks read t of the
+// chunk from LDS with the rolling update (a read too long for the buffer is
+// walked from global memory, alone in its chunk).  Records are in file
+// order, so a chunk's bytes are one contiguous range.  The buffer holds the
+// base code of every byte, not the byte: bits 0-1 of the code in plane A
+// (32 bytes per word) and, on the main path, bit 2 in plane B (32 bytes per
+// word), 3/8 of the raw size.
+constexpr int L1_SB = 32768;  // staged bytes per chunk
 
 struct ChunkLds {
-  ull sb[L1_SB / 8];
-  ull base;   // first staged byte (8-byte aligned)
-  int m;      // reads in the chunk
-  int lds;    // bytes are staged
+  ull A[L1_SB / 32];       // code bits 0-1 of byte 32 w + i at bits 2i, 2i+1 of A[w]
+  uint32_t B[L1_SB / 32];  // code bit 2 of byte 32 w + i at bit i of B[w] (main path)
+  ull base;                // first staged byte (32-byte aligned)
+  int m;                   // reads in the chunk
+  int lds;                 // bytes are staged
 };
 
-// Sets up the chunk starting at read c0 (< r1) and stages its bytes.
+// Sets up the chunk starting at read c0 (< r1) and stages its codes.
+template <int ENC>
 __device__ __forceinline__ void stage_chunk(ChunkLds& C, const uint8_t* fa, uint64_t n, const ull* rstart,
                                             const uint32_t* rlen, uint64_t c0, uint64_t r1, int NT) {
   const uint64_t r = c0 + threadIdx.x;
-  const ull first = rstart[c0] & ~7ull;
+  const ull first = rstart[c0] & ~31ull;
   int fits = 0;
   if (r < r1 && (int)threadIdx.x < NT) fits = rstart[r] + rlen[r] - first <= (ull)L1_SB;
   // records ascend in the file: the reads that fit are a prefix
@@ -520,8 +524,23 @@ __device__ __forceinline__ void stage_chunk(ChunkLds& C, const uint8_t* fa, uint
   if (C.lds) {
     const uint64_t last = c0 + C.m - 1;
     const ull end = rstart[last] + rlen[last];
-    const ull nw = (end - first + 7) / 8;
-    for (ull i = threadIdx.x; i < nw; i += NT) C.sb[i] = load_word8(fa, n, first + 8 * i);
+    const ull nw = (end - first + 31) / 32;
+    for (ull i = threadIdx.x; i < nw; i += NT) {
+      ull a = 0;
+      uint32_t b = 0;
+#pragma unroll
+      for (int h = 0; h < 4; h++) {
+        const ull w = load_word8(fa, n, first + 32 * i + 8 * h);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          const uint32_t el = base_code<ENC>((uint32_t)(w >> (8 * k)) & 0xFFu);
+          a |= (ull)(el & 3u) << (16 * h + 2 * k);
+          b |= (el >> 2) << (8 * h + k);
+        }
+      }
+      C.A[i] = a;
+      if (ENC == ENC_MAIN) C.B[i] = b;
+    }
   }
   __syncthreads();
 }
@@ -605,16 +624,19 @@ __device__ __forceinline__ int walk_read(const ChunkLds& C, const uint8_t* fa, u
                                          Emit&& emit) {
   int q = 0;
   if (C.lds) {
-    // 8 staged bytes per LDS load
+    // 32 staged codes per pair of LDS loads
     while (q < Q && i < len) {
       const ull off = s + i - C.base;
-      ull w = C.sb[off >> 3] >> ((off & 7) * 8);
-      int nb = 8 - (int)(off & 7);
+      const uint32_t sh = (uint32_t)(off & 31);
+      ull a = C.A[off >> 5] >> (2 * sh);
+      uint32_t b = ENC == ENC_MAIN ? C.B[off >> 5] >> sh : 0u;
+      int nb = 32 - (int)sh;
       while (nb > 0 && q < Q && i < len) {
-        const uint32_t el = base_code<ENC>((uint32_t)w & 0xFFu);
+        const uint32_t el = ((uint32_t)a & 3u) | ((b & 1u) << 2);
         km = ((km << 2) | (Key)el) & mask;
         const bool own = own_step<Key, OWN>(os, cf, km, el, i, K);
-        w >>= 8;
+        a >>= 2;
+        b >>= 1;
         nb--;
         i++;
         if (i >= (uint32_t)K && own) emit(q++, km);
@@ -665,7 +687,7 @@ __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n
   const uint64_t r1 = r0 + rpb < R ? r0 + rpb : R;
   const int bpos = KB - d1;
   for (uint64_t c0 = r0; c0 < r1;) {
-    stage_chunk(C, fa, n, rstart, rlen, c0, r1, L1_NT);
+    stage_chunk<ENC>(C, fa, n, rstart, rlen, c0, r1, L1_NT);
     const int m = C.m;
     if ((int)threadIdx.x < m) {
       const uint64_t r = c0 + threadIdx.x;
@@ -706,7 +728,7 @@ __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_
   const int bpos = KB - d1;
   // stage_chunk's barriers order the set-up above before the first walk
   for (uint64_t c0 = r0; c0 < r1;) {
-    stage_chunk(C, fa, n, rstart, rlen, c0, r1, L1_NT);
+    stage_chunk<ENC>(C, fa, n, rstart, rlen, c0, r1, L1_NT);
     const int m = C.m;
     if ((int)threadIdx.x < m) {
       const uint64_t r = c0 + threadIdx.x;
