@@ -494,25 +494,37 @@ struct TileLds {
 // order, so a chunk's bytes are one contiguous range.  The buffer holds the
 // base code of every byte, not the byte: bits 0-1 of the code in plane A
 // (32 bytes per word) and, on the main path, bit 2 in plane B (32 bytes per
-// word), 3/8 of the raw size.
-constexpr int L1_SB = 32768;  // staged bytes per chunk
+// word), 3/8 of the raw size.  Its size (sb bytes, in dynamic LDS) is chosen
+// per input so that 256 typical records fit.
+constexpr int L1_SB_MIN = 8192, L1_SB_MAX = 98304;  // staged bytes per chunk
 
 struct ChunkLds {
-  ull A[L1_SB / 32];       // code bits 0-1 of byte 32 w + i at bits 2i, 2i+1 of A[w]
-  uint32_t B[L1_SB / 32];  // code bit 2 of byte 32 w + i at bit i of B[w] (main path)
-  ull base;                // first staged byte (32-byte aligned)
-  int m;                   // reads in the chunk
-  int lds;                 // bytes are staged
+  ull base;  // first staged byte (32-byte aligned)
+  int m;     // reads in the chunk
+  int lds;   // bytes are staged
 };
+
+struct Stage {
+  ull* A;       // code bits 0-1 of byte 32 w + i at bits 2i, 2i+1 of A[w]
+  uint32_t* B;  // code bit 2 of byte 32 w + i at bit i of B[w] (main path)
+  int sb;       // capacity in bytes (a multiple of 32)
+};
+
+// Bytes of dynamic LDS of a level-1 kernel: the two planes, then nd digit
+// entries of dig_bytes each.
+__host__ __device__ constexpr uint32_t l1_lds_bytes(int sb, int nd, int dig_bytes) {
+  return (uint32_t)(sb / 32) * 12u + (uint32_t)nd * (uint32_t)dig_bytes;
+}
 
 // Sets up the chunk starting at read c0 (< r1) and stages its codes.
 template <int ENC>
-__device__ __forceinline__ void stage_chunk(ChunkLds& C, const uint8_t* fa, uint64_t n, const ull* rstart,
-                                            const uint32_t* rlen, uint64_t c0, uint64_t r1, int NT) {
+__device__ __forceinline__ void stage_chunk(ChunkLds& C, Stage sg, const uint8_t* fa, uint64_t n,
+                                            const ull* rstart, const uint32_t* rlen, uint64_t c0, uint64_t r1,
+                                            int NT) {
   const uint64_t r = c0 + threadIdx.x;
   const ull first = rstart[c0] & ~31ull;
   int fits = 0;
-  if (r < r1 && (int)threadIdx.x < NT) fits = rstart[r] + rlen[r] - first <= (ull)L1_SB;
+  if (r < r1 && (int)threadIdx.x < NT) fits = rstart[r] + rlen[r] - first <= (ull)sg.sb;
   // records ascend in the file: the reads that fit are a prefix
   const int m = (int)block_sum<L1_NT>((ull)fits);
   if (threadIdx.x == 0) {
@@ -538,8 +550,8 @@ __device__ __forceinline__ void stage_chunk(ChunkLds& C, const uint8_t* fa, uint
           b |= (el >> 2) << (8 * h + k);
         }
       }
-      C.A[i] = a;
-      if (ENC == ENC_MAIN) C.B[i] = b;
+      sg.A[i] = a;
+      if (ENC == ENC_MAIN) sg.B[i] = b;
     }
   }
   __syncthreads();
@@ -619,7 +631,8 @@ __device__ __forceinline__ bool own_step(OwnState& o, const OwnCfg& cf, Key km, 
 }
 
 template <class Key, int ENC, bool OWN, class Emit>
-__device__ __forceinline__ int walk_read(const ChunkLds& C, const uint8_t* fa, ull s, uint32_t len, uint32_t& i,
+__device__ __forceinline__ int walk_read(const ChunkLds& C, Stage sg, const uint8_t* fa, ull s, uint32_t len,
+                                         uint32_t& i,
                                          Key& km, OwnState& os, const OwnCfg& cf, int K, Key mask, int Q,
                                          Emit&& emit) {
   int q = 0;
@@ -628,8 +641,8 @@ __device__ __forceinline__ int walk_read(const ChunkLds& C, const uint8_t* fa, u
     while (q < Q && i < len) {
       const ull off = s + i - C.base;
       const uint32_t sh = (uint32_t)(off & 31);
-      ull a = C.A[off >> 5] >> (2 * sh);
-      uint32_t b = ENC == ENC_MAIN ? C.B[off >> 5] >> sh : 0u;
+      ull a = sg.A[off >> 5] >> (2 * sh);
+      uint32_t b = ENC == ENC_MAIN ? sg.B[off >> 5] >> sh : 0u;
       int nb = 32 - (int)sh;
       while (nb > 0 && q < Q && i < len) {
         const uint32_t el = ((uint32_t)a & 3u) | ((b & 1u) << 2);
@@ -677,24 +690,27 @@ __global__ void k_lmer_freq(const uint8_t* fa, const ull* rstart, const uint32_t
 template <class Key, int ENC, bool OWN>
 __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n, const ull* rstart,
                                                    const uint32_t* rlen, uint64_t R, int K, int KB,
-                                                   Key mask, int d1, int rpb, OwnCfg cf, uint32_t* H) {
+                                                   Key mask, int d1, int rpb, int sb, OwnCfg cf,
+                                                   uint32_t* H) {
   __shared__ ChunkLds C;
-  extern __shared__ uint32_t l1_dyn[];
-  uint32_t* h = l1_dyn;  // 2^d1 digit counts (dynamic LDS)
+  // dynamic LDS: plane A, plane B, 2^d1 digit counts
+  extern __shared__ ull l1_dyn[];
+  const Stage sg{l1_dyn, reinterpret_cast<uint32_t*>(l1_dyn + sb / 32), sb};
+  uint32_t* h = sg.B + sb / 32;
   const int nb = 1 << d1;
   for (int i = threadIdx.x; i < nb; i += L1_NT) h[i] = 0;
   const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
   const uint64_t r1 = r0 + rpb < R ? r0 + rpb : R;
   const int bpos = KB - d1;
   for (uint64_t c0 = r0; c0 < r1;) {
-    stage_chunk<ENC>(C, fa, n, rstart, rlen, c0, r1, L1_NT);
+    stage_chunk<ENC>(C, sg, fa, n, rstart, rlen, c0, r1, L1_NT);
     const int m = C.m;
     if ((int)threadIdx.x < m) {
       const uint64_t r = c0 + threadIdx.x;
       uint32_t i = 0;
       Key km = 0;
       OwnState os{0, 0, 0, 0};
-      walk_read<Key, ENC, OWN>(C, fa, rstart[r], rlen[r], i, km, os, cf, K, mask, 0x7fffffff,
+      walk_read<Key, ENC, OWN>(C, sg, fa, rstart[r], rlen[r], i, km, os, cf, K, mask, 0x7fffffff,
                                [&](int, Key x) { atomicAdd(&h[key_digit<Key>(x, bpos, d1)], 1u); });
     }
     c0 += m;
@@ -713,12 +729,14 @@ template <class Key, int ENC, bool OWN>
 __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_t n,
                                                       const ull* rstart, const uint32_t* rlen,
                                                       uint64_t R, int K, int KB, Key mask, int d1,
-                                                      int rpb, OwnCfg cf, const ull* O, Key* S1) {
+                                                      int rpb, int sb, OwnCfg cf, const ull* O, Key* S1) {
   __shared__ ChunkLds C;
-  extern __shared__ ull l1_dyn64[];
-  ull* base = l1_dyn64;                                     // 2^d1 output cursors
-  uint32_t* h = reinterpret_cast<uint32_t*>(l1_dyn64 + (1 << d1));  // 2^d1 ranks
+  // dynamic LDS: plane A, 2^d1 output cursors, plane B, 2^d1 ranks
+  extern __shared__ ull l1_dyn[];
   const int nb = 1 << d1;
+  ull* base = l1_dyn + sb / 32;
+  const Stage sg{l1_dyn, reinterpret_cast<uint32_t*>(base + nb), sb};
+  uint32_t* h = sg.B + sb / 32;
   for (int i = threadIdx.x; i < nb; i += L1_NT) {
     base[i] = O[(ull)i * gridDim.x + blockIdx.x];
     h[i] = 0;
@@ -728,14 +746,14 @@ __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_
   const int bpos = KB - d1;
   // stage_chunk's barriers order the set-up above before the first walk
   for (uint64_t c0 = r0; c0 < r1;) {
-    stage_chunk<ENC>(C, fa, n, rstart, rlen, c0, r1, L1_NT);
+    stage_chunk<ENC>(C, sg, fa, n, rstart, rlen, c0, r1, L1_NT);
     const int m = C.m;
     if ((int)threadIdx.x < m) {
       const uint64_t r = c0 + threadIdx.x;
       uint32_t i = 0;
       Key km = 0;
       OwnState os{0, 0, 0, 0};
-      walk_read<Key, ENC, OWN>(C, fa, rstart[r], rlen[r], i, km, os, cf, K, mask, 0x7fffffff,
+      walk_read<Key, ENC, OWN>(C, sg, fa, rstart[r], rlen[r], i, km, os, cf, K, mask, 0x7fffffff,
                                [&](int, Key x) {
                                  const uint32_t d = key_digit<Key>(x, bpos, d1);
                                  S1[base[d] + atomicAdd(&h[d], 1u)] = x;
@@ -2378,6 +2396,13 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
                                ? tun.l1_chunks
                                : (int)std::min<uint64_t>(8, std::max<uint64_t>(1, R / (L1_NT * 1024ull))));
   const uint64_t nblk = (R + rpb - 1) / rpb;
+  // LDS staging: room for 256 records of the input's mean size, + 25 %
+  int sb;
+  {
+    const double want = 256.0 * ((double)n / (double)R) * 1.25;
+    sb = (int)std::min<double>(L1_SB_MAX, std::max<double>(L1_SB_MIN, want));
+    sb = (sb + 1023) & ~1023;
+  }
   ull *boff, *O;
   uint32_t* H;
   Key* S2;
@@ -2390,8 +2415,7 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   c->timing.bucket_bits = btot;
   PK_HIP(c, hipEventRecord(c->ev[2], st));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_hist<Key, ENC, OWN>), dim3((unsigned)nblk), dim3(L1_NT),
-                     (unsigned)(B1 * sizeof(uint32_t)), st,
-                     fa, n, rstart, rlen, R, K, KB, mask, d1, rpb, cf, H);
+                     l1_lds_bytes(sb, (int)B1, 4), st, fa, n, rstart, rlen, R, K, KB, mask, d1, rpb, sb, cf, H);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[3], st));
   PK_TRY(scan_excl<uint32_t>(c, H, O, B1 * nblk));
@@ -2415,8 +2439,8 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[4], st));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key, ENC, OWN>), dim3((unsigned)nblk), dim3(L1_NT),
-                     (unsigned)(B1 * (sizeof(ull) + sizeof(uint32_t))), st, fa, n, rstart, rlen, R, K, KB,
-                     mask, d1, rpb, cf, O, S);
+                     l1_lds_bytes(sb, (int)B1, 12), st, fa, n, rstart, rlen, R, K, KB, mask, d1, rpb, sb, cf,
+                     O, S);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[16], st));
   uint64_t B = B1;

@@ -123,6 +123,22 @@ def test_edge_inputs(ctx, oracle_mod, fa):
         _check_table(ctx, oracle_mod, fa, k)
 
 
+def test_mixed_read_lengths(ctx, oracle_mod):
+    """Reads from 20 bp to 150 kbp in one input: the level-1 chunks stage
+    their bytes in LDS sized for the mean record, and a read longer than the
+    staging buffer is walked from global memory, alone in its chunk."""
+    _p("case: mixed read lengths")
+    rng = np.random.default_rng(11)
+    lens = [20, 150, 40_000, 90, 150_000, 33, 64, 65, 2_000, 100_000] + [100] * 300 + [70_000]
+    recs = []
+    for i, n in enumerate(lens):
+        seq = rng.choice(np.frombuffer(b"ACGTACGTN", dtype=np.uint8), size=n)
+        recs.append(b">r%d\n" % i + seq.tobytes() + b"\n")
+    fa = b"".join(recs)
+    for k in (21, 32, 64):
+        _check_table(ctx, oracle_mod, fa, k)
+
+
 def test_skewed_buckets(ctx, oracle_mod):
     """Identical and near-identical k-mers overflow a bucket's capacity and
     go through the split path."""
