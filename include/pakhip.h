@@ -97,6 +97,7 @@ typedef struct pk_timing {
   int32_t bucket_bits;  /* partition width used by the last pk_count */
   int32_t n_split;      /* buckets that needed splitting */
   float l1_scatter_ms;  /* first scatter level alone (one kernel, part of scatter_ms) */
+  float normalize_ms;   /* input-format normalization (pk_count_format; 0 otherwise) */
 } pk_timing;
 
 /* Create a context on HIP device `device`.  hbm_budget_bytes = 0 means no
@@ -133,6 +134,34 @@ int pk_ctx_release_buffers(pk_ctx* ctx);
  * resident.  8 <= k <= 64.  On success *out describes the table. */
 int pk_count(pk_ctx* ctx, const char* fasta, uint64_t nbytes, int k, int fasta_on_device,
              pk_table* out);
+
+/* Input formats beyond the reference's strict single-line FASTA (no
+ * reference counterpart: the reference reads only that format,
+ * src/input_reading.cpp:69-312 and src/kmer_counting.cpp:2920-2945).  The
+ * device rewrites the input into strict single-line FASTA, then counts it as
+ * pk_count does; the table is pk_count's table of the rewritten input.
+ *   PK_FMT_FASTA_STRICT  the reference's format, passed through unchanged
+ *   PK_FMT_FASTA         multi-line FASTA: a record is a line starting with
+ *                        '>' and the lines up to the next such line, whose
+ *                        concatenation is the read; lines before the first
+ *                        record are ignored
+ *   PK_FMT_FASTQ         four-line FASTQ records ('@' header, sequence, '+'
+ *                        line, qualities); the header becomes '>' + the rest
+ *                        of the line and the sequence the read
+ * In both rewritten formats a '\r' just before a '\n' is dropped, and every
+ * record ends with '\n'.  A FASTQ input whose line count is not a multiple of
+ * 4, or whose header / separator line does not start with '@' / '+', fails
+ * with PK_EFORMAT. */
+#define PK_FMT_FASTA_STRICT 0
+#define PK_FMT_FASTA 1
+#define PK_FMT_FASTQ 2
+int pk_count_format(pk_ctx* ctx, const char* data, uint64_t nbytes, int k, int format, int data_on_device,
+                    pk_table* out);
+/* The rewrite alone: the strict single-line FASTA stays in the context
+ * (*out_bytes long) for pk_download_normalized. */
+int pk_normalize(pk_ctx* ctx, const char* data, uint64_t nbytes, int format, int data_on_device,
+                 uint64_t* out_bytes);
+int pk_download_normalized(pk_ctx* ctx, char* host_out);
 
 /* One rank's share of the counted table at nranks ranks: the k-mers whose
  * owner is `rank`, i.e. the kmer_proc_buf that rank holds after the

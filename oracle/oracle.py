@@ -193,3 +193,42 @@ def visit_value(count: int, coverage: int) -> int:
 def ref_binary(k: int) -> str | None:
     p = os.path.join(REF_DIR, f"pakref_k{k}")
     return p if os.path.exists(p) else None
+
+
+def normalize(data, fmt: str) -> bytes:
+    """Input-format rewrite into strict single-line FASTA, restated line by
+    line on the CPU from the format rules of pk_count_format (include/
+    pakhip.h).  The reference has no counterpart: it reads only strict
+    single-line FASTA (src/input_reading.cpp:69-312, src/kmer_counting.cpp:
+    2920-2945).  fmt: "fasta" (multi-line) or "fastq" (four-line records).
+    Raises ValueError where the device reports PK_EFORMAT."""
+    data = bytes(data)
+    if not data:
+        return b""
+    lines = data.split(b"\n")
+    if lines[-1] == b"":
+        lines.pop()  # the empty tail after a final '\n' is not a line
+    lines = [ln[:-1] if ln.endswith(b"\r") else ln for ln in lines]
+    out = []
+    if fmt == "fastq":
+        if len(lines) % 4:
+            raise ValueError("FASTQ line count is not a multiple of 4")
+        for j in range(0, len(lines), 4):
+            h, seq, plus = lines[j], lines[j + 1], lines[j + 2]
+            if not h.startswith(b"@") or not plus.startswith(b"+"):
+                raise ValueError("malformed FASTQ record")
+            out.append(b">" + h[1:] + b"\n" + seq + b"\n")
+        return b"".join(out)
+    if fmt != "fasta":
+        raise ValueError(fmt)
+    header, seq = None, []
+    for ln in lines:
+        if ln.startswith(b">"):
+            if header is not None:
+                out.append(header + b"\n" + b"".join(seq) + b"\n")
+            header, seq = ln, []
+        elif header is not None:
+            seq.append(ln)
+    if header is not None:
+        out.append(header + b"\n" + b"".join(seq) + b"\n")
+    return b"".join(out)

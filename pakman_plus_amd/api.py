@@ -47,7 +47,7 @@ class _Timing(ctypes.Structure):
                 ("cutoff_ms", ctypes.c_float), ("mn_ms", ctypes.c_float),
                 ("n_instances", ctypes.c_uint64), ("n_reads", ctypes.c_uint64),
                 ("bucket_bits", ctypes.c_int32), ("n_split", ctypes.c_int32),
-                ("l1_scatter_ms", ctypes.c_float)]
+                ("l1_scatter_ms", ctypes.c_float), ("normalize_ms", ctypes.c_float)]
 
 
 class Tuning(ctypes.Structure):
@@ -85,6 +85,7 @@ MNODE_DTYPE = np.dtype([("key_lo", "<u8"), ("key_hi", "<u8"), ("pre_cnt", "<i4",
 # every symbol declared in include/pakhip.h
 EXPORTS = ("pk_ctx_create", "pk_ctx_destroy", "pk_last_error", "pk_ctx_stream",
            "pk_ctx_device_bytes", "pk_ctx_set_tuning", "pk_ctx_release_buffers", "pk_count",
+           "pk_count_format", "pk_normalize", "pk_download_normalized",
            "pk_count_rank_share",
            "pk_lmer_frequency", "pk_builder_count",
            "pk_cutoff", "pk_cutoff_threshold", "pk_kept_table", "pk_macro_nodes", "pk_mn_wiring", "pk_download_table", "pk_download_kept",
@@ -94,6 +95,9 @@ EXPORTS = ("pk_ctx_create", "pk_ctx_destroy", "pk_last_error", "pk_ctx_stream",
            "pk_server_last_error")
 
 _LIB = None
+
+# input formats of pk_count_format (PK_FMT_* in include/pakhip.h)
+FORMATS = {"fasta-strict": 0, "fasta": 1, "fastq": 2}
 
 
 def load_library():
@@ -124,6 +128,12 @@ def load_library():
     L.pk_ctx_device_bytes.restype = u64
     L.pk_count.argtypes = [vp, vp, u64, i32, i32, ctypes.POINTER(_Table)]
     L.pk_count.restype = i32
+    L.pk_count_format.argtypes = [vp, vp, u64, i32, i32, i32, ctypes.POINTER(_Table)]
+    L.pk_count_format.restype = i32
+    L.pk_normalize.argtypes = [vp, vp, u64, i32, i32, ctypes.POINTER(u64)]
+    L.pk_normalize.restype = i32
+    L.pk_download_normalized.argtypes = [vp, vp]
+    L.pk_download_normalized.restype = i32
     L.pk_count_rank_share.argtypes = [vp, vp, u64, i32, i32, i32, i32, ctypes.POINTER(_Table)]
     L.pk_count_rank_share.restype = i32
     L.pk_lmer_frequency.argtypes = [vp, vp, u64, i32, vp]
@@ -257,6 +267,34 @@ class Context:
         self.k = k
         self.table = TableView(int(t.n), int(t.words), int(t.k), t.keys or 0, t.counts or 0)
         return self.table
+
+    def count_format(self, data, k: int, fmt: str = "fasta", device_ptr: int | None = None,
+                     nbytes: int | None = None) -> TableView:
+        """Count k-mers of multi-line FASTA ("fasta"), four-line FASTQ
+        ("fastq") or strict single-line FASTA ("fasta-strict"), '\r\n' line
+        ends allowed in the first two: the table of pk_count on the input
+        rewritten as strict single-line FASTA (see normalize)."""
+        f = FORMATS[fmt]
+        t = _Table()
+        if device_ptr is not None:
+            rc = self._lib.pk_count_format(self._h, ctypes.c_void_p(device_ptr), nbytes, k, f, 1, ctypes.byref(t))
+        else:
+            a = _as_u8(data)
+            rc = self._lib.pk_count_format(self._h, a.ctypes.data, a.nbytes, k, f, 0, ctypes.byref(t))
+        self._check(rc)
+        self.k = k
+        self.table = TableView(int(t.n), int(t.words), int(t.k), t.keys or 0, t.counts or 0)
+        return self.table
+
+    def normalize(self, data, fmt: str = "fasta") -> bytes:
+        """The input rewritten on the device as strict single-line FASTA."""
+        a = _as_u8(data)
+        nb = ctypes.c_uint64(0)
+        self._check(self._lib.pk_normalize(self._h, a.ctypes.data, a.nbytes, FORMATS[fmt], 0, ctypes.byref(nb)))
+        out = np.empty(int(nb.value), dtype=np.uint8)
+        if nb.value:
+            self._check(self._lib.pk_download_normalized(self._h, out.ctypes.data))
+        return out.tobytes()
 
     def count_rank_share(self, fasta, k: int, nranks: int, rank: int, device_ptr: int | None = None,
                          nbytes: int | None = None) -> TableView:

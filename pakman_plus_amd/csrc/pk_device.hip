@@ -327,6 +327,93 @@ __global__ __launch_bounds__(NL_NT) void k_nl_write(const uint8_t* fa, uint64_t 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Input formats: multi-line FASTA and four-line FASTQ (either with '\r\n'
+// line ends) rewritten into the strict single-line FASTA the counting path
+// reads.  Line j of the input is [a, b): a = 0 or one past the (j-1)-th
+// '\n', b = the j-th '\n' or the end; a '\r' at b - 1 is dropped.  One
+// thread per line computes the line's output length, a scan places the
+// lines, and one wave per line copies it.
+
+constexpr int FMT_FASTA = 1;
+constexpr int FMT_FASTQ = 2;
+
+__device__ __forceinline__ void fmt_line(const uint8_t* in, uint64_t n, const ull* nl, uint64_t m, uint64_t j,
+                                         ull& a, ull& b) {
+  a = j ? nl[j - 1] + 1 : 0;
+  b = j < m ? nl[j] : n;
+  if (b > a && in[b - 1] == '\r') b--;
+}
+
+// Multi-line FASTA: *hmin = the first line that starts with '>'.
+__global__ void k_fmt_first_record(const uint8_t* in, uint64_t n, const ull* nl, uint64_t m, uint64_t L,
+                                   ull* hmin) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= L) return;
+  const ull a = j ? nl[j - 1] + 1 : 0;
+  if (a < n && in[a] == '>') atomicMin(hmin, (ull)j);
+}
+
+// Output bytes of every line.  FASTA: nothing before the first record; a
+// header line gives itself + '\n'; a sequence line gives itself; the last
+// line of a record adds the '\n' that ends the read (a header with no
+// sequence line gives an empty read).  FASTQ: header (its '@' becomes '>')
+// and sequence lines give themselves + '\n'; the '+' and quality lines give
+// nothing.  bad receives the first malformed FASTQ line, too_long the first
+// line whose output exceeds 32 bits.
+template <int FMT>
+__global__ void k_fmt_lines(const uint8_t* in, uint64_t n, const ull* nl, uint64_t m, uint64_t L,
+                            const ull* hmin, uint32_t* olen, ull* bad, ull* too_long) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= L) return;
+  ull a, b;
+  fmt_line(in, n, nl, m, j, a, b);
+  const ull len = b - a;
+  ull out;
+  if (FMT == FMT_FASTA) {
+    if (j < *hmin) {
+      out = 0;
+    } else {
+      const bool header = len > 0 && in[a] == '>';
+      bool last = j + 1 >= L;
+      if (!last) {
+        const ull a2 = nl[j] + 1;
+        last = a2 < n && in[a2] == '>';
+      }
+      out = len + (header ? 1 : 0) + (last ? 1 : 0);
+    }
+  } else {
+    const int role = (int)(j & 3);
+    if ((role == 0 && !(len > 0 && in[a] == '@')) || (role == 2 && !(len > 0 && in[a] == '+')))
+      atomicMin(bad, (ull)j);
+    out = role < 2 ? len + 1 : 0;
+  }
+  if (out > 0xFFFFFFFFull) {
+    atomicMin(too_long, (ull)j);
+    out = 0;
+  }
+  olen[j] = (uint32_t)out;
+}
+
+// One wave per line: the line's bytes, then the '\n's its length adds.
+template <int FMT>
+__global__ __launch_bounds__(256) void k_fmt_copy(const uint8_t* in, uint64_t n, const ull* nl, uint64_t m,
+                                                  uint64_t L, const uint32_t* olen, const ull* ooff,
+                                                  uint8_t* out) {
+  const uint64_t j = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
+  if (j >= L) return;
+  const uint32_t ol = olen[j];
+  if (ol == 0) return;
+  ull a, b;
+  fmt_line(in, n, nl, m, j, a, b);
+  const uint32_t len = (uint32_t)(b - a);
+  const ull o = ooff[j];
+  const bool fq_header = FMT == FMT_FASTQ && (j & 3) == 0;
+  for (uint32_t i = lane; i < len; i += 64) out[o + i] = (fq_header && i == 0) ? (uint8_t)'>' : in[a + i];
+  for (uint32_t i = len + lane; i < ol; i += 64) out[o + i] = '\n';
+}
+
 // Builder-API records (Concurrent_Kmer_Partial_Counter::
 // default_count_and_process, concurrent_kmer_counter.hpp:2018-2082): from
 // the current position the scan finds the next skip symbol ('>'), skips to
@@ -1978,7 +2065,7 @@ enum BufId {
   B_FASTA, B_BLK, B_BLKOFF, B_NL, B_RSTART, B_RLEN, B_KC, B_KOFS, B_SMALL, B_BCNT, B_BOFF, B_BCUR,
   B_S, B_C, B_TU, B_UOFF, B_TK, B_TC, B_HIST, B_KBS, B_KBOFF, B_KK, B_KC2, B_MNQ, B_MNPLAN,
   B_MNNT, B_MNNG, B_MNNOFF, B_MNFLAG, B_NODES, B_TMP, B_OVF, B_PT0, B_PT1, B_PT2, B_PT3, B_PT4,
-  B_PT5, B_PT6, B_BLKGT, B_S2, B_BCNT2, B_BOFF2, B_BCUR2, B_TP, B_HIST1, B_OFF1, B_HIST2, B_OFF2, B_CUTBH, B_LFREQ, B_WORDER, B_WIRE, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
+  B_PT5, B_PT6, B_BLKGT, B_S2, B_BCNT2, B_BOFF2, B_BCUR2, B_TP, B_HIST1, B_OFF1, B_HIST2, B_OFF2, B_CUTBH, B_LFREQ, B_WORDER, B_WIRE, B_NORM, B_NORMNL, B_NORMLEN, B_NORMOFF, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
 };
 
 }  // namespace
@@ -1990,7 +2077,7 @@ struct pk_ctx {
   uint64_t held = 0;
   std::string err;
   DevBuf buf[B_COUNT];
-  hipEvent_t ev[18] = {};
+  hipEvent_t ev[20] = {};
   pk_timing timing{};
   // state of the current table
   int k = 0;
@@ -2001,6 +2088,9 @@ struct pk_ctx {
   int key_bits = 0;  // bits of a table key: 2k (main path) or the word (builder)
   // stages whose event times have not been read yet
   bool pend_count = false, pend_split = false, pend_cutoff = false, pend_mn = false;
+  bool pend_norm = false;  // normalize_ms of the last pk_count_format
+  uint64_t norm_n = 0;     // bytes of the last rewritten input (B_NORM)
+  bool have_norm = false;
   pk_tuning tuning{};  // zeros: built-in choices
 
   int fail(int code, const std::string& what) {
@@ -2242,7 +2332,7 @@ int resolve_overflow(pk_ctx* c, Key* S, Key* R, int32_t* C, int K, int fixed_bit
 // Index of the event bytes ('\n', or '\n' and '>') of fa: *m events at
 // *ev_out.  small[0] receives the position of the first '>'.
 int event_index(pk_ctx* c, const uint8_t* fa, uint64_t n, uint8_t e0, uint8_t e1, ull* small,
-                ull** ev_out, uint64_t* m_out, uint64_t* p0_out) {
+                ull** ev_out, uint64_t* m_out, uint64_t* p0_out, int ev_buf = B_NL) {
   hipStream_t st = c->stream;
   const uint64_t nb = (n + NL_PER_BLOCK - 1) / NL_PER_BLOCK;
   ull *blk, *blkoff, *nl, *blkgt;
@@ -2261,7 +2351,7 @@ int event_index(pk_ctx* c, const uint8_t* fa, uint64_t n, uint8_t e0, uint8_t e1
   PK_HIP(c, hipMemcpyAsync(&hm[1], small, 8, hipMemcpyDeviceToHost, st));
   PK_HIP(c, hipStreamSynchronize(st));
   const uint64_t m = hm[0];
-  PK_TRY(need(c, B_NL, m + 1, &nl));
+  PK_TRY(need(c, ev_buf, m + 1, &nl));
   if (nb) {
     hipLaunchKernelGGL(k_nl_write, dim3((unsigned)nb), dim3(NL_NT), 0, st, fa, n, e0, e1, blkoff, nl);
     PK_HIP(c, hipGetLastError());
@@ -2269,6 +2359,65 @@ int event_index(pk_ctx* c, const uint8_t* fa, uint64_t n, uint8_t e0, uint8_t e1
   *ev_out = nl;
   *m_out = m;
   *p0_out = hm[1] == ~0ull ? n : hm[1];
+  return PK_OK;
+}
+
+// The input rewritten into strict single-line FASTA (B_NORM, *out_n bytes).
+int normalize_impl(pk_ctx* c, const uint8_t* in, uint64_t n, int fmt, const uint8_t** out_p, uint64_t* out_n) {
+  hipStream_t st = c->stream;
+  ull* small;
+  PK_TRY(need(c, B_SMALL, 16, &small));
+  // small[8] = first '>' (unused), [9] = first record line, [10] = first
+  // malformed line, [11] = first over-long line
+  PK_TRY(fill_u64(c, small + 8, ~0ull, 4));
+  ull* nl;
+  uint64_t m, p0;
+  PK_TRY(event_index(c, in, n, '\n', '\n', small + 8, &nl, &m, &p0, B_NORMNL));
+  // a last line without '\n' is a line; the empty tail after a final '\n' is not
+  ull last_nl = 0;
+  if (m) PK_TRY(d2h(c, &last_nl, nl + m - 1, 8));
+  const uint64_t L = m + ((n > 0 && (m == 0 || last_nl + 1 < n)) ? 1 : 0);
+  if (fmt == FMT_FASTQ && L % 4 != 0)
+    return c->fail(PK_EFORMAT, "FASTQ input has " + std::to_string(L) + " lines, not a multiple of 4");
+  uint32_t* olen;
+  ull* ooff;
+  uint8_t* out;
+  PK_TRY(need(c, B_NORMLEN, L + 1, &olen));
+  PK_TRY(need(c, B_NORMOFF, L + 1, &ooff));
+  PK_TRY(need(c, B_NORM, n + L + 64, &out));
+  if (L) {
+    if (fmt == FMT_FASTA) {
+      hipLaunchKernelGGL(k_fmt_first_record, dim3(grid_for(L, 256)), dim3(256), 0, st, in, n, nl, m, L, small + 9);
+      PK_HIP(c, hipGetLastError());
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fmt_lines<FMT_FASTA>), dim3(grid_for(L, 256)), dim3(256), 0, st, in, n,
+                         nl, m, L, small + 9, olen, small + 10, small + 11);
+    } else {
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fmt_lines<FMT_FASTQ>), dim3(grid_for(L, 256)), dim3(256), 0, st, in, n,
+                         nl, m, L, small + 9, olen, small + 10, small + 11);
+    }
+    PK_HIP(c, hipGetLastError());
+  }
+  PK_TRY(scan_excl<uint32_t>(c, olen, ooff, L));
+  ull hv[3];
+  PK_HIP(c, hipMemcpyAsync(&hv[0], ooff + L, 8, hipMemcpyDeviceToHost, st));
+  PK_HIP(c, hipMemcpyAsync(&hv[1], small + 10, 16, hipMemcpyDeviceToHost, st));
+  PK_HIP(c, hipStreamSynchronize(st));
+  if (hv[1] != ~0ull)
+    return c->fail(PK_EFORMAT, "FASTQ record line " + std::to_string(hv[1]) +
+                                   (hv[1] % 4 == 0 ? " does not start with '@'" : " does not start with '+'"));
+  if (hv[2] != ~0ull) return c->fail(PK_EINVAL, "line " + std::to_string(hv[2]) + " longer than 4 GiB");
+  if (L) {
+    const uint64_t nb = (L + 3) / 4;
+    if (fmt == FMT_FASTA)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fmt_copy<FMT_FASTA>), dim3((unsigned)nb), dim3(256), 0, st, in, n, nl,
+                         m, L, olen, ooff, out);
+    else
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fmt_copy<FMT_FASTQ>), dim3((unsigned)nb), dim3(256), 0, st, in, n, nl,
+                         m, L, olen, ooff, out);
+    PK_HIP(c, hipGetLastError());
+  }
+  *out_p = out;
+  *out_n = hv[0];
   return PK_OK;
 }
 
@@ -2590,7 +2739,7 @@ template <class Key>
 int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
   c->have_table = c->have_kept = c->have_nodes = false;
   c->timing = pk_timing{};
-  c->pend_count = c->pend_split = c->pend_cutoff = c->pend_mn = false;
+  c->pend_count = c->pend_split = c->pend_cutoff = c->pend_mn = c->pend_norm = false;
   c->key_bits = 2 * K;
   PK_HIP(c, hipEventRecord(c->ev[0], c->stream));
   ull* rstart;
@@ -2625,7 +2774,7 @@ template <class Key>
 int rank_share_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K, int P, int rank) {
   c->have_table = c->have_kept = c->have_nodes = false;
   c->timing = pk_timing{};
-  c->pend_count = c->pend_split = c->pend_cutoff = c->pend_mn = false;
+  c->pend_count = c->pend_split = c->pend_cutoff = c->pend_mn = c->pend_norm = false;
   c->key_bits = 2 * K;
   PK_HIP(c, hipEventRecord(c->ev[0], c->stream));
   ull* freq;
@@ -2643,7 +2792,7 @@ int builder_count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int klen, uint6
                        BldSyms sy) {
   c->have_table = c->have_kept = c->have_nodes = false;
   c->timing = pk_timing{};
-  c->pend_count = c->pend_split = c->pend_cutoff = c->pend_mn = false;
+  c->pend_count = c->pend_split = c->pend_cutoff = c->pend_mn = c->pend_norm = false;
   c->key_bits = (int)sizeof(Key) * 8;
   PK_HIP(c, hipEventRecord(c->ev[0], c->stream));
   ull* rstart;
@@ -2898,6 +3047,70 @@ static int fasta_to_device(pk_ctx* c, const char* fasta, uint64_t nbytes, int fa
   return PK_OK;
 }
 
+// The rewrite of pk_normalize / pk_count_format; events 17-18 bracket it.
+static int normalize_input(pk_ctx* c, const char* data, uint64_t nbytes, int format, int on_device,
+                           const uint8_t** out, uint64_t* out_n) {
+  if (format != PK_FMT_FASTA_STRICT && format != PK_FMT_FASTA && format != PK_FMT_FASTQ)
+    return c->fail(PK_EINVAL, "unknown input format " + std::to_string(format));
+  const uint8_t* in;
+  PK_TRY(fasta_to_device(c, data, nbytes, on_device, &in));
+  c->have_norm = false;
+  PK_HIP(c, hipEventRecord(c->ev[17], c->stream));
+  if (format == PK_FMT_FASTA_STRICT) {
+    uint8_t* d;
+    PK_TRY(need(c, B_NORM, nbytes + 64, &d));
+    if (nbytes) PK_HIP(c, hipMemcpyAsync(d, in, nbytes, hipMemcpyDeviceToDevice, c->stream));
+    *out = d;
+    *out_n = nbytes;
+  } else {
+    PK_TRY(normalize_impl(c, in, nbytes, format, out, out_n));
+  }
+  PK_HIP(c, hipEventRecord(c->ev[18], c->stream));
+  c->norm_n = *out_n;
+  c->have_norm = true;
+  return PK_OK;
+}
+
+int pk_normalize(pk_ctx* c, const char* data, uint64_t nbytes, int format, int data_on_device,
+                 uint64_t* out_bytes) {
+  if (!c || !out_bytes || (!data && nbytes)) return PK_EINVAL;
+  PK_HIP(c, hipSetDevice(c->device));
+  const uint8_t* norm;
+  uint64_t nn;
+  PK_TRY(normalize_input(c, data, nbytes, format, data_on_device, &norm, &nn));
+  c->timing.normalize_ms = 0.f;
+  c->pend_norm = true;
+  *out_bytes = nn;
+  return PK_OK;
+}
+
+int pk_download_normalized(pk_ctx* c, char* host_out) {
+  if (!c || !host_out) return PK_EINVAL;
+  if (!c->have_norm) return c->fail(PK_ESTATE, "no rewritten input");
+  if (c->norm_n == 0) return PK_OK;
+  return d2h(c, host_out, c->buf[B_NORM].p, c->norm_n);
+}
+
+int pk_count_format(pk_ctx* c, const char* data, uint64_t nbytes, int k, int format, int data_on_device,
+                    pk_table* out) {
+  if (!c || !out || (!data && nbytes)) return PK_EINVAL;
+  if (k < 8 || k > 64) return c->fail(PK_EINVAL, "k must be in [8, 64]");
+  PK_HIP(c, hipSetDevice(c->device));
+  if (format == PK_FMT_FASTA_STRICT) return pk_count(c, data, nbytes, k, data_on_device, out);
+  const uint8_t* norm;
+  uint64_t nn;
+  PK_TRY(normalize_input(c, data, nbytes, format, data_on_device, &norm, &nn));
+  int rc = (k <= 32) ? count_impl<uint64_t>(c, norm, nn, k) : count_impl<u128>(c, norm, nn, k);
+  if (rc) return rc;
+  c->pend_norm = true;
+  out->n = c->U;
+  out->words = c->words;
+  out->k = c->k;
+  out->keys = reinterpret_cast<const uint64_t*>(c->buf[B_TK].p);
+  out->counts = reinterpret_cast<const int32_t*>(c->buf[B_TC].p);
+  return PK_OK;
+}
+
 int pk_count_rank_share(pk_ctx* c, const char* fasta, uint64_t nbytes, int k, int nranks, int rank,
                         int fasta_on_device, pk_table* out) {
   if (!c || !out || (!fasta && nbytes)) return PK_EINVAL;
@@ -3058,7 +3271,7 @@ int pk_last_timing(const pk_ctx* cc, pk_timing* out) {
     (void)hipEventElapsedTime(&v, c->ev[a], c->ev[b]);
     return v;
   };
-  if (c->pend_count || c->pend_cutoff || c->pend_mn) {
+  if (c->pend_count || c->pend_cutoff || c->pend_mn || c->pend_norm) {
     PK_HIP(c, hipSetDevice(c->device));
     PK_HIP(c, hipStreamSynchronize(c->stream));
   }
@@ -3079,6 +3292,10 @@ int pk_last_timing(const pk_ctx* cc, pk_timing* out) {
   if (c->pend_mn) {
     c->timing.mn_ms = el(14, 15);
     c->pend_mn = false;
+  }
+  if (c->pend_norm) {
+    c->timing.normalize_ms = el(17, 18);
+    c->pend_norm = false;
   }
   *out = c->timing;
   return PK_OK;
