@@ -8,11 +8,12 @@ line (metric, value in k-mer instances per second, roofline of the dominant
 kernel, CPU baseline of the oracle on a bounded sample).
 
   python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload bacterial]
-                  [--forms plain,batched,served] [--sets 256] [--slots 4] [--load 0.8]
+                  [--forms plain,batched,served,formats] [--sets 256] [--slots 4] [--load 0.8]
 
 The batched and served forms run after the plain timed steps, on the
 256-set workload (see bench_batched_served), and are reported in the same
-JSON line under "batched" and "served".
+JSON line under "batched" and "served"; "formats" times the same reads as
+FASTQ with '\r\n' line ends (bench_formats).
 """
 from __future__ import annotations
 
@@ -217,6 +218,66 @@ def bench_batched_served(args):
     return out
 
 
+def fastq_crlf(fa_bytes: bytes) -> bytes:
+    """The reads of a strict single-line FASTA as four-line FASTQ records
+    with '\r\n' line ends (constant quality line)."""
+    lines = fa_bytes.split(b"\n")
+    out = []
+    for i in range(0, len(lines) - 1, 2):
+        seq = lines[i + 1]
+        out.append(b"@" + lines[i][1:] + b"\r\n" + seq + b"\r\n+\r\n" + b"I" * len(seq) + b"\r\n")
+    return b"".join(out)
+
+
+def bench_formats(fa, k, t, steps, warmup, tun, plain_ms):
+    """pk_count_format on the same reads written as FASTQ with '\r\n' line
+    ends, resident in HBM: the device rewrite alone (pk_normalize) and whole
+    steps (pk_count_format + pk_cutoff + pk_macro_nodes)."""
+    import numpy as np
+    import torch
+    from pakman_plus_amd import api
+    fq = np.frombuffer(fastq_crlf(fa.tobytes()), dtype=np.uint8)
+    dev = torch.from_numpy(fq).to("cuda:0")
+    torch.cuda.synchronize()
+    ctx = api.Context(0)
+    if tun:
+        ctx.set_tuning(**tun)
+    import ctypes
+    nb = ctypes.c_uint64(0)
+
+    def normalize():
+        ctx._check(ctx._lib.pk_normalize(ctx._h, ctypes.c_void_p(dev.data_ptr()), int(fq.nbytes),
+                                         api.FORMATS["fastq"], 1, ctypes.byref(nb)))
+        return ctx.timing()["normalize_ms"]
+
+    def step():
+        ctx.count_format(None, k, "fastq", device_ptr=dev.data_ptr(), nbytes=int(fq.nbytes))
+        ctx.cutoff(t)
+        ctx.macro_nodes()
+        return ctx.timing()
+
+    for _ in range(warmup):
+        normalize()
+        step()
+    norm = sorted(normalize() for _ in range(steps))[steps // 2]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tm = step()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / steps
+    n_in, n_out = int(fq.nbytes), int(nb.value)
+    ctx.close()
+    del dev
+    return {"workload": "the bacterial reads as four-line FASTQ records with \\r\\n line ends",
+            "input_bytes": n_in, "rewritten_bytes": n_out,
+            "normalize_ms": norm,
+            "normalize_alg_bytes": n_in + n_out,
+            "normalize_frac_of_peak": (n_in + n_out) / (norm * 1e-3) / HBM_PEAK,
+            "step_ms": el * 1e3, "value": int(tm["n_instances"]) / el, "unit": "k-mers/s",
+            "plain_step_ms": plain_ms}
+
+
 def load_traffic(stage, workload):
     """HBM bytes per launch of the stage's kernel, from the rocprofv3 PMC runs
     (FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a plain copy of
@@ -252,8 +313,9 @@ def main():
     ap.add_argument("--cpu-frac", type=float, default=0.25)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--forms", default="plain,batched,served",
-                    help="plain, plus the batched/served forms on the 256-set workload")
+    ap.add_argument("--forms", default="plain,batched,served,formats",
+                    help="plain, plus the batched/served forms on the 256-set workload and the "
+                         "FASTQ form of the bacterial reads")
     ap.add_argument("--sets", type=int, default=256)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--load", type=float, default=0.8)
@@ -348,6 +410,8 @@ def main():
     del dev
     ctx.close()
     forms = set(args.forms.split(","))
+    if "formats" in forms and args.workload == "bacterial":
+        line["formats"] = bench_formats(fa, k, args.t, args.steps, args.warmup, tun, ms_per_step)
     if forms & {"batched", "served"}:
         line.update(bench_batched_served(args))
     if not args.no_cpu_baseline:

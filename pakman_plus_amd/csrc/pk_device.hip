@@ -271,6 +271,32 @@ __device__ __forceinline__ void scan16(const uint8_t* fa, uint64_t n, uint64_t b
   }
 }
 
+// scan16 that also returns the 16 bytes (zero past n).
+__device__ __forceinline__ uint4 scan16_keep(const uint8_t* in, uint64_t n, uint64_t base, uint32_t e0,
+                                             uint32_t& ev) {
+  ev = 0;
+  uint4 v = make_uint4(0, 0, 0, 0);
+  if (base + 16 <= n && ((reinterpret_cast<uintptr_t>(in + base) & 15) == 0)) {
+    v = *reinterpret_cast<const uint4*>(in + base);
+    ev = match4(v.x, e0) | (match4(v.y, e0) << 4) | (match4(v.z, e0) << 8) | (match4(v.w, e0) << 12);
+  } else {
+    uint32_t w[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 16; i++) {
+      if (base + i >= n) break;
+      const uint32_t b = in[base + i];
+      w[i >> 2] |= b << (8 * (i & 3));
+      ev |= (uint32_t)(b == e0) << i;
+    }
+    v = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint8_t byte_of(const uint4& v, int i) {
+  const uint32_t w = i < 8 ? (i < 4 ? v.x : v.y) : (i < 12 ? v.z : v.w);
+  return (uint8_t)(w >> (8 * (i & 3)));
+}
+
 // Event bytes of the index: bytes equal to e0 or e1 ('\n' and '\n' for the
 // line index of the main path; delimiter and skip symbol for the builder).
 // Per block: the number of event bytes and the first '>'.
@@ -363,12 +389,15 @@ __global__ void k_fmt_first_record(const uint8_t* in, uint64_t n, const ull* nl,
 // line whose output exceeds 32 bits.
 template <int FMT>
 __global__ void k_fmt_lines(const uint8_t* in, uint64_t n, const ull* nl, uint64_t m, uint64_t L,
-                            const ull* hmin, uint32_t* olen, ull* bad, ull* too_long) {
+                            const ull* hmin, uint32_t* olen, ull* lstart, uint32_t* llen, ull* bad,
+                            ull* too_long) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= L) return;
   ull a, b;
   fmt_line(in, n, nl, m, j, a, b);
   const ull len = b - a;
+  lstart[j] = a;
+  llen[j] = (uint32_t)(len < 0xFFFFFFFFull ? len : 0xFFFFFFFFull);
   ull out;
   if (FMT == FMT_FASTA) {
     if (j < *hmin) {
@@ -395,23 +424,72 @@ __global__ void k_fmt_lines(const uint8_t* in, uint64_t n, const ull* nl, uint64
   olen[j] = (uint32_t)out;
 }
 
-// One wave per line: the line's bytes, then the '\n's its length adds.
+// The copy, byte-parallel in the layout of k_nl_write: lane t of slab j
+// holds 16 input bytes, and the block scan of the '\n' counts gives the
+// line of its first byte (blk_nl = '\n's before each block).  A content
+// byte of line i goes to ooff[i] + (its offset in the line); the '\n' that
+// ends line i (or the input's last byte, for a last line without one) writes
+// the '\n's the line's output length adds after its content.  Sixteen
+// content bytes of one line (most chunks) take one 16-byte store.
 template <int FMT>
-__global__ __launch_bounds__(256) void k_fmt_copy(const uint8_t* in, uint64_t n, const ull* nl, uint64_t m,
-                                                  uint64_t L, const uint32_t* olen, const ull* ooff,
-                                                  uint8_t* out) {
-  const uint64_t j = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const uint32_t lane = threadIdx.x & 63;
-  if (j >= L) return;
-  const uint32_t ol = olen[j];
-  if (ol == 0) return;
-  ull a, b;
-  fmt_line(in, n, nl, m, j, a, b);
-  const uint32_t len = (uint32_t)(b - a);
-  const ull o = ooff[j];
-  const bool fq_header = FMT == FMT_FASTQ && (j & 3) == 0;
-  for (uint32_t i = lane; i < len; i += 64) out[o + i] = (fq_header && i == 0) ? (uint8_t)'>' : in[a + i];
-  for (uint32_t i = len + lane; i < ol; i += 64) out[o + i] = '\n';
+__global__ __launch_bounds__(NL_NT) void k_fmt_copy(const uint8_t* __restrict__ in, uint64_t n,
+                                                    const ull* __restrict__ blk_nl,
+                                                    const ull* __restrict__ lstart,
+                                                    const uint32_t* __restrict__ llen,
+                                                    const uint32_t* __restrict__ olen,
+                                                    const ull* __restrict__ ooff, uint8_t* __restrict__ out) {
+  const uint64_t b0 = (uint64_t)blockIdx.x * NL_PER_BLOCK + (uint64_t)threadIdx.x * 16;
+  uint32_t ev[NL_SLABS];
+  uint4 raw[NL_SLABS];
+  ull packed = 0;
+#pragma unroll
+  for (int j = 0; j < NL_SLABS; j++) {
+    raw[j] = scan16_keep(in, n, b0 + j * NL_SLAB, '\n', ev[j]);
+    packed |= (ull)__popc(ev[j]) << (16 * j);
+  }
+  ull tot;
+  const ull ex = block_scan_total<NL_NT>(packed, &tot);
+  const ull blk_base = blk_nl[blockIdx.x];
+  // slab: 16 bytes v at pos0 whose '\n's are evj, the first in line `line`
+  auto slab = [&](uint64_t pos0, uint4 v, uint32_t evj, uint64_t line) {
+    if (pos0 >= n) return;
+    const int cnt = n - pos0 < 16 ? (int)(n - pos0) : 16;
+    // the line's fields: start, content length, output length and place
+    ull a = lstart[line], oo = ooff[line];
+    uint32_t len = llen[line], ol = olen[line];
+    if (cnt == 16 && evj == 0 && pos0 + 16 <= a + len && pos0 != n - 16) {
+      // (a byte-aligned 16-byte copy: one unaligned dwordx4 store)
+      if (ol && !(FMT == FMT_FASTQ && (line & 3) == 0 && pos0 == a)) __builtin_memcpy(out + oo + (pos0 - a), &v, 16);
+      return;
+    }
+    for (int i = 0; i < cnt; i++) {
+      const uint64_t pos = pos0 + i;
+      const bool is_nl = (evj >> i) & 1u;
+      if (ol) {
+        if (pos < a + len) {
+          const bool gt = FMT == FMT_FASTQ && (line & 3) == 0 && pos == a;
+          out[oo + (pos - a)] = gt ? (uint8_t)'>' : byte_of(v, i);
+        }
+        if (is_nl || pos == n - 1) {
+          // the end of the line: the '\n's after its content
+          for (uint32_t q = len; q < ol; q++) out[oo + q] = '\n';
+        }
+      }
+      if (is_nl && pos + 1 < n) {
+        line++;
+        a = lstart[line];
+        oo = ooff[line];
+        len = llen[line];
+        ol = olen[line];
+      }
+    }
+  };
+  static_assert(NL_SLABS == 4, "four slabs");
+  const ull t0 = tot & 0xFFFFull, t1 = (tot >> 16) & 0xFFFFull, t2 = (tot >> 32) & 0xFFFFull;
+  slab(b0, raw[0], ev[0], blk_base + (ex & 0xFFFFull));
+  slab(b0 + NL_SLAB, raw[1], ev[1], blk_base + t0 + ((ex >> 16) & 0xFFFFull));
+  slab(b0 + 2 * NL_SLAB, raw[2], ev[2], blk_base + t0 + t1 + ((ex >> 32) & 0xFFFFull));
+  slab(b0 + 3 * NL_SLAB, raw[3], ev[3], blk_base + t0 + t1 + t2 + ((ex >> 48) & 0xFFFFull));
 }
 
 // Builder-API records (Concurrent_Kmer_Partial_Counter::
@@ -2065,7 +2143,7 @@ enum BufId {
   B_FASTA, B_BLK, B_BLKOFF, B_NL, B_RSTART, B_RLEN, B_KC, B_KOFS, B_SMALL, B_BCNT, B_BOFF, B_BCUR,
   B_S, B_C, B_TU, B_UOFF, B_TK, B_TC, B_HIST, B_KBS, B_KBOFF, B_KK, B_KC2, B_MNQ, B_MNPLAN,
   B_MNNT, B_MNNG, B_MNNOFF, B_MNFLAG, B_NODES, B_TMP, B_OVF, B_PT0, B_PT1, B_PT2, B_PT3, B_PT4,
-  B_PT5, B_PT6, B_BLKGT, B_S2, B_BCNT2, B_BOFF2, B_BCUR2, B_TP, B_HIST1, B_OFF1, B_HIST2, B_OFF2, B_CUTBH, B_LFREQ, B_WORDER, B_WIRE, B_NORM, B_NORMNL, B_NORMLEN, B_NORMOFF, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
+  B_PT5, B_PT6, B_BLKGT, B_S2, B_BCNT2, B_BOFF2, B_BCUR2, B_TP, B_HIST1, B_OFF1, B_HIST2, B_OFF2, B_CUTBH, B_LFREQ, B_WORDER, B_WIRE, B_NORM, B_NORMNL, B_NORMLEN, B_NORMOFF, B_NORMLS, B_NORMLL, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
 };
 
 }  // namespace
@@ -2379,21 +2457,23 @@ int normalize_impl(pk_ctx* c, const uint8_t* in, uint64_t n, int fmt, const uint
   const uint64_t L = m + ((n > 0 && (m == 0 || last_nl + 1 < n)) ? 1 : 0);
   if (fmt == FMT_FASTQ && L % 4 != 0)
     return c->fail(PK_EFORMAT, "FASTQ input has " + std::to_string(L) + " lines, not a multiple of 4");
-  uint32_t* olen;
-  ull* ooff;
+  uint32_t *olen, *llen;
+  ull *ooff, *lstart;
   uint8_t* out;
   PK_TRY(need(c, B_NORMLEN, L + 1, &olen));
   PK_TRY(need(c, B_NORMOFF, L + 1, &ooff));
+  PK_TRY(need(c, B_NORMLS, L + 1, &lstart));
+  PK_TRY(need(c, B_NORMLL, L + 1, &llen));
   PK_TRY(need(c, B_NORM, n + L + 64, &out));
   if (L) {
     if (fmt == FMT_FASTA) {
       hipLaunchKernelGGL(k_fmt_first_record, dim3(grid_for(L, 256)), dim3(256), 0, st, in, n, nl, m, L, small + 9);
       PK_HIP(c, hipGetLastError());
       hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fmt_lines<FMT_FASTA>), dim3(grid_for(L, 256)), dim3(256), 0, st, in, n,
-                         nl, m, L, small + 9, olen, small + 10, small + 11);
+                         nl, m, L, small + 9, olen, lstart, llen, small + 10, small + 11);
     } else {
       hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fmt_lines<FMT_FASTQ>), dim3(grid_for(L, 256)), dim3(256), 0, st, in, n,
-                         nl, m, L, small + 9, olen, small + 10, small + 11);
+                         nl, m, L, small + 9, olen, lstart, llen, small + 10, small + 11);
     }
     PK_HIP(c, hipGetLastError());
   }
@@ -2407,13 +2487,15 @@ int normalize_impl(pk_ctx* c, const uint8_t* in, uint64_t n, int fmt, const uint
                                    (hv[1] % 4 == 0 ? " does not start with '@'" : " does not start with '+'"));
   if (hv[2] != ~0ull) return c->fail(PK_EINVAL, "line " + std::to_string(hv[2]) + " longer than 4 GiB");
   if (L) {
-    const uint64_t nb = (L + 3) / 4;
+    // '\n's before each index block: the scan event_index left in B_BLKOFF
+    const ull* blk_nl = reinterpret_cast<const ull*>(c->buf[B_BLKOFF].p);
+    const uint64_t nb = (n + NL_PER_BLOCK - 1) / NL_PER_BLOCK;
     if (fmt == FMT_FASTA)
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fmt_copy<FMT_FASTA>), dim3((unsigned)nb), dim3(256), 0, st, in, n, nl,
-                         m, L, olen, ooff, out);
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fmt_copy<FMT_FASTA>), dim3((unsigned)nb), dim3(NL_NT), 0, st, in, n,
+                         blk_nl, lstart, llen, olen, ooff, out);
     else
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fmt_copy<FMT_FASTQ>), dim3((unsigned)nb), dim3(256), 0, st, in, n, nl,
-                         m, L, olen, ooff, out);
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fmt_copy<FMT_FASTQ>), dim3((unsigned)nb), dim3(NL_NT), 0, st, in, n,
+                         blk_nl, lstart, llen, olen, ooff, out);
     PK_HIP(c, hipGetLastError());
   }
   *out_p = out;

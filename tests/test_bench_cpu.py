@@ -39,3 +39,12 @@ def test_tuning_lookup():
     assert tune.for_size(entries, 5000, 32) == {}
     table = tune.load()
     assert table and {e["key_words"] for e in table} == {1, 2}
+
+
+def test_bench_fastq_writer_matches_test_writer():
+    """bench.fastq_crlf writes the records tests/format_util.to_fastq does."""
+    import bench
+    from format_util import to_fastq
+    from golden_util import read_fasta
+    fa = read_fasta("syn_tiny.fa.gz")
+    assert bench.fastq_crlf(fa) == to_fastq(fa, b"\r\n")
