@@ -43,6 +43,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pakhip.h"
@@ -2166,6 +2167,10 @@ struct pk_ctx {
   int key_bits = 0;  // bits of a table key: 2k (main path) or the word (builder)
   // stages whose event times have not been read yet
   bool pend_count = false, pend_split = false, pend_cutoff = false, pend_mn = false;
+  // pinned staging of host input: PIN_THREADS host threads, two buffers each
+  void* pin[8] = {};
+  hipEvent_t pin_ev[8] = {};
+  bool pin_used[8] = {};
   bool pend_norm = false;  // normalize_ms of the last pk_count_format
   uint64_t norm_n = 0;     // bytes of the last rewritten input (B_NORM)
   bool have_norm = false;
@@ -3057,6 +3062,10 @@ void pk_ctx_destroy(pk_ctx* c) {
     if (b.p) (void)hipFree(b.p);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (int i = 0; i < 8; i++) {
+    if (c->pin[i]) (void)hipHostFree(c->pin[i]);
+    if (c->pin_ev[i]) (void)hipEventDestroy(c->pin_ev[i]);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -3092,20 +3101,16 @@ int pk_ctx_release_buffers(pk_ctx* c) {
   return PK_OK;
 }
 
+static int fasta_to_device(pk_ctx* c, const char* fasta, uint64_t nbytes, int fasta_on_device,
+                           const uint8_t** fa);
+
 int pk_count(pk_ctx* c, const char* fasta, uint64_t nbytes, int k, int fasta_on_device,
              pk_table* out) {
   if (!c || !out || (!fasta && nbytes)) return PK_EINVAL;
   if (k < 8 || k > 64) return c->fail(PK_EINVAL, "k must be in [8, 64]");
   PK_HIP(c, hipSetDevice(c->device));
   const uint8_t* fa;
-  if (fasta_on_device) {
-    fa = reinterpret_cast<const uint8_t*>(fasta);
-  } else {
-    uint8_t* d;
-    PK_TRY(need(c, B_FASTA, nbytes + 64, &d));
-    if (nbytes) PK_HIP(c, hipMemcpyAsync(d, fasta, nbytes, hipMemcpyHostToDevice, c->stream));
-    fa = d;
-  }
+  PK_TRY(fasta_to_device(c, fasta, nbytes, fasta_on_device, &fa));
   int rc = (k <= 32) ? count_impl<uint64_t>(c, fa, nbytes, k) : count_impl<u128>(c, fa, nbytes, k);
   if (rc) return rc;
   out->n = c->U;
@@ -3113,6 +3118,59 @@ int pk_count(pk_ctx* c, const char* fasta, uint64_t nbytes, int k, int fasta_on_
   out->k = c->k;
   out->keys = reinterpret_cast<const uint64_t*>(c->buf[B_TK].p);
   out->counts = reinterpret_cast<const int32_t*>(c->buf[B_TC].p);
+  return PK_OK;
+}
+
+// Host input to the device through pinned staging: PIN_THREADS host threads
+// each take a contiguous slice and copy it PIN_CHUNK bytes at a time into
+// one of their two pinned buffers, then queue the DMA copy on the context
+// stream; a buffer is refilled once the event after its last copy has
+// completed.  A DMA copy from pageable memory runs at about half the rate of
+// one from pinned memory (27 against 56 GB/s measured on an MI355X host), so
+// the staging pays for its host memcpy.  Small inputs and pinned inputs take
+// one direct copy.
+constexpr int PIN_THREADS = 4;
+constexpr uint64_t PIN_CHUNK = 8ull << 20;
+constexpr uint64_t PIN_MIN = 16ull << 20;
+
+static int h2d_staged(pk_ctx* c, uint8_t* dst, const char* src, uint64_t n) {
+  // pinned (page-locked) input is copied as it is
+  hipPointerAttribute_t attr{};
+  const bool pinned = hipPointerGetAttributes(&attr, src) == hipSuccess && attr.type == hipMemoryTypeHost;
+  (void)hipGetLastError();  // an unregistered pointer may leave an error behind
+  if (n < PIN_MIN || pinned) {
+    PK_HIP(c, hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream));
+    return PK_OK;
+  }
+  for (int i = 0; i < 2 * PIN_THREADS; i++) {
+    if (!c->pin[i]) PK_HIP(c, hipHostMalloc(&c->pin[i], PIN_CHUNK, hipHostMallocDefault));
+    if (!c->pin_ev[i]) PK_HIP(c, hipEventCreateWithFlags(&c->pin_ev[i], hipEventDisableTiming));
+  }
+  const uint64_t slice = ((n + PIN_THREADS - 1) / PIN_THREADS + 63) & ~63ull;
+  hipError_t err[PIN_THREADS];
+  auto work = [&](int t) {
+    err[t] = hipSuccess;
+    const uint64_t lo = std::min<uint64_t>(n, (uint64_t)t * slice), hi = std::min<uint64_t>(n, lo + slice);
+    hipError_t e = hipSetDevice(c->device);
+    int k = 0;
+    for (uint64_t off = lo; off < hi && e == hipSuccess; off += PIN_CHUNK, k ^= 1) {
+      const int b = 2 * t + k;
+      const uint64_t len = std::min<uint64_t>(PIN_CHUNK, hi - off);
+      if (c->pin_used[b]) e = hipEventSynchronize(c->pin_ev[b]);
+      if (e != hipSuccess) break;
+      std::memcpy(c->pin[b], src + off, len);
+      e = hipMemcpyAsync(dst + off, c->pin[b], len, hipMemcpyHostToDevice, c->stream);
+      if (e == hipSuccess) e = hipEventRecord(c->pin_ev[b], c->stream);
+      c->pin_used[b] = true;
+    }
+    err[t] = e;
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < PIN_THREADS; t++) th.emplace_back(work, t);
+  work(0);
+  for (auto& x : th) x.join();
+  for (int t = 0; t < PIN_THREADS; t++)
+    if (err[t] != hipSuccess) return c->hip_fail(err[t], "staged host-to-device copy");
   return PK_OK;
 }
 
@@ -3124,7 +3182,7 @@ static int fasta_to_device(pk_ctx* c, const char* fasta, uint64_t nbytes, int fa
   }
   uint8_t* d;
   PK_TRY(need(c, B_FASTA, nbytes + 64, &d));
-  if (nbytes) PK_HIP(c, hipMemcpyAsync(d, fasta, nbytes, hipMemcpyHostToDevice, c->stream));
+  if (nbytes) PK_TRY(h2d_staged(c, d, fasta, nbytes));
   *fa = d;
   return PK_OK;
 }
@@ -3235,14 +3293,7 @@ int pk_builder_count(pk_ctx* c, const char* fasta, uint64_t nbytes, int klen, ui
   const BldSyms sy{(uint8_t)delimiter, (uint8_t)skip_symbol};
   PK_HIP(c, hipSetDevice(c->device));
   const uint8_t* fa;
-  if (fasta_on_device) {
-    fa = reinterpret_cast<const uint8_t*>(fasta);
-  } else {
-    uint8_t* d;
-    PK_TRY(need(c, B_FASTA, nbytes + 64, &d));
-    if (nbytes) PK_HIP(c, hipMemcpyAsync(d, fasta, nbytes, hipMemcpyHostToDevice, c->stream));
-    fa = d;
-  }
+  PK_TRY(fasta_to_device(c, fasta, nbytes, fasta_on_device, &fa));
   int rc = kmer_bits == 64 ? builder_count_impl<uint64_t>(c, fa, nbytes, klen, read_len, sy)
                            : builder_count_impl<u128>(c, fa, nbytes, klen, read_len, sy);
   if (rc) return rc;

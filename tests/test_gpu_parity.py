@@ -139,6 +139,28 @@ def test_mixed_read_lengths(ctx, oracle_mod):
         _check_table(ctx, oracle_mod, fa, k)
 
 
+def test_host_input_paths(ctx, gen_mod):
+    """A 33 MB host FASTA goes through the pinned staging buffers when it is
+    pageable and is copied directly when it is pinned or already on the
+    device: the three tables are identical."""
+    import torch
+    _p("case: host input paths")
+    fa = gen_mod.make_fasta(gen_mod.config_small())
+    assert fa.nbytes > (16 << 20)
+    ctx.count(fa, 32)
+    k1, c1 = ctx.table_host()
+    pinned = torch.empty(fa.nbytes, dtype=torch.uint8, pin_memory=True)
+    pinned.numpy()[:] = fa
+    ctx.count(pinned.numpy(), 32)
+    k2, c2 = ctx.table_host()
+    dev = torch.from_numpy(fa).to("cuda:0")
+    torch.cuda.synchronize()
+    ctx.count(None, 32, device_ptr=dev.data_ptr(), nbytes=fa.nbytes)
+    k3, c3 = ctx.table_host()
+    assert np.array_equal(k1, k2) and np.array_equal(c1, c2)
+    assert np.array_equal(k1, k3) and np.array_equal(c1, c3)
+
+
 def test_skewed_buckets(ctx, oracle_mod):
     """Identical and near-identical k-mers overflow a bucket's capacity and
     go through the split path."""
