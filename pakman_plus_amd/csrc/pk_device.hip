@@ -3142,9 +3142,22 @@ static int h2d_staged(pk_ctx* c, uint8_t* dst, const char* src, uint64_t n) {
     PK_HIP(c, hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream));
     return PK_OK;
   }
-  for (int i = 0; i < 2 * PIN_THREADS; i++) {
-    if (!c->pin[i]) PK_HIP(c, hipHostMalloc(&c->pin[i], PIN_CHUNK, hipHostMallocDefault));
-    if (!c->pin_ev[i]) PK_HIP(c, hipEventCreateWithFlags(&c->pin_ev[i], hipEventDisableTiming));
+  bool ready = true;
+  for (int i = 0; i < 2 * PIN_THREADS && ready; i++) {
+    if (!c->pin[i] && hipHostMalloc(&c->pin[i], PIN_CHUNK, hipHostMallocDefault) != hipSuccess) {
+      c->pin[i] = nullptr;
+      ready = false;
+    }
+    if (ready && !c->pin_ev[i] && hipEventCreateWithFlags(&c->pin_ev[i], hipEventDisableTiming) != hipSuccess) {
+      c->pin_ev[i] = nullptr;
+      ready = false;
+    }
+  }
+  if (!ready) {
+    // no pinned memory to be had: one direct copy
+    (void)hipGetLastError();
+    PK_HIP(c, hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream));
+    return PK_OK;
   }
   const uint64_t slice = ((n + PIN_THREADS - 1) / PIN_THREADS + 63) & ~63ull;
   hipError_t err[PIN_THREADS];
