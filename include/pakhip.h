@@ -113,10 +113,11 @@ uint64_t pk_ctx_device_bytes(const pk_ctx* ctx);
  * them).  Zero fields keep the built-in choice.
  *   l1_chunks      reads per level-1 partition block, in chunks of 256 (1..64;
  *                  built-in: 1..8 by input size)
- *   l1_bits        level-1 radix digit width (1..10; built-in 8)
+ *   l1_bits        level-1 radix digit width (1..PK_TUNING_L1_BITS_MAX; built-in 8)
  *   bucket_factor  instances per final bucket as a multiple of the LDS hash
  *                  table's key capacity, i.e. the inverse hash load target
  *                  (built-in 3.0) */
+#define PK_TUNING_L1_BITS_MAX 10
 typedef struct pk_tuning {
   int32_t l1_chunks;
   int32_t l1_bits;
@@ -311,7 +312,9 @@ int pk_server_download(pk_server* srv, uint64_t ticket, uint64_t* keys, int32_t*
 /* Per-size tuning for the jobs of a server: a job of nbytes with key words
  * w (1 for k <= 32, else 2) uses the first entry (ascending max_bytes) with
  * nbytes <= max_bytes and key_words == w (or 0 = any), else the built-in
- * choices.  Copies the table; n == 0 clears it. */
+ * choices.  Copies the table; n == 0 clears it.  Entries with key_words
+ * outside 0..2 or tuning values outside pk_ctx_set_tuning's ranges fail
+ * with PK_EINVAL and leave the table as it was. */
 typedef struct pk_tune_entry {
   uint64_t max_bytes;
   int32_t key_words;

@@ -629,7 +629,7 @@ __device__ __forceinline__ uint32_t base_code(uint32_t c) {
 
 constexpr int L1_NT = 256;
 constexpr int MS_NT = 512;
-constexpr int L1_MAX_BITS = 10;     // level-1 digit width (direct scatter)
+constexpr int L1_MAX_BITS = PK_TUNING_L1_BITS_MAX;  // level-1 digit width (direct scatter)
 constexpr int L1_DEFAULT_BITS = 8;  // wider digits keep more partial lines open
 constexpr int MS_MAX_BITS = 8;   // further levels (tile split into runs)
 
@@ -3081,7 +3081,7 @@ int pk_ctx_set_tuning(pk_ctx* c, const pk_tuning* t) {
     return PK_OK;
   }
   if (t->l1_chunks < 0 || t->l1_chunks > 64 || t->l1_bits < 0 || t->l1_bits > L1_MAX_BITS ||
-      t->bucket_factor < 0.f || t->bucket_factor > 64.f)
+      !(t->bucket_factor >= 0.f && t->bucket_factor <= 64.f))
     return c->fail(PK_EINVAL, "tuning values out of range");
   c->tuning = *t;
   return PK_OK;

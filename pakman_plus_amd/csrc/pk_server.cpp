@@ -373,6 +373,13 @@ int pk_server_download(pk_server* s, uint64_t ticket, uint64_t* keys, int32_t* c
 
 int pk_server_set_tuning(pk_server* s, const pk_tune_entry* entries, int n) {
   if (!s || n < 0 || (n && !entries)) return PK_EINVAL;
+  for (int i = 0; i < n; i++) {
+    const pk_tune_entry& e = entries[i];
+    const pk_tuning& t = e.tuning;
+    if (e.key_words < 0 || e.key_words > 2 || t.l1_chunks < 0 || t.l1_chunks > 64 || t.l1_bits < 0 ||
+        t.l1_bits > PK_TUNING_L1_BITS_MAX || !(t.bucket_factor >= 0.f && t.bucket_factor <= 64.f))
+      return s->fail(PK_EINVAL, "tuning entry " + std::to_string(i) + " out of range");
+  }
   std::vector<pk_tune_entry> t(entries, entries + n);
   std::sort(t.begin(), t.end(),
             [](const pk_tune_entry& a, const pk_tune_entry& b) { return a.max_bytes < b.max_bytes; });

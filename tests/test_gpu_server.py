@@ -159,3 +159,17 @@ def test_admission_budget_serialises(gen_mod):
         for r in res:
             assert r.status in (api.PK_OK, api.PK_EEMPTY) and r.est_bytes > 1
             srv.release(r.ticket)
+
+
+def test_server_tuning_entries_checked():
+    """Out-of-range tuning entries are refused as a whole."""
+    from pakman_plus_amd import api
+    _p("case: server tuning entries")
+    with api.Server(0, n_slots=1) as srv:
+        good = {"max_bytes": 1 << 30, "key_words": 1, "l1_chunks": 2, "l1_bits": 10, "bucket_factor": 2.2}
+        srv.set_tuning([good])
+        for bad in ({"l1_bits": 11}, {"l1_chunks": 65}, {"key_words": 3}, {"bucket_factor": float("nan")}):
+            with pytest.raises(api.PakHipError) as e:
+                srv.set_tuning([good, dict(good, **bad)])
+            assert e.value.code == api.PK_EINVAL
+        srv.set_tuning([])
