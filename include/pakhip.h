@@ -111,7 +111,7 @@ void* pk_ctx_stream(pk_ctx* ctx);
 uint64_t pk_ctx_device_bytes(const pk_ctx* ctx);
 /* Launch-shape choices of the counting kernels (results do not depend on
  * them).  Zero fields keep the built-in choice.
- *   l1_chunks      reads per level-1 partition block, in chunks of 256 (1..64;
+ *   l1_chunks      reads per level-1 partition block, in chunks of 512 (1..64;
  *                  built-in: 1..8 by input size)
  *   l1_bits        level-1 radix digit width (1..PK_TUNING_L1_BITS_MAX; built-in 8)
  *   bucket_factor  instances per final bucket as a multiple of the LDS hash

@@ -627,7 +627,7 @@ __device__ __forceinline__ uint32_t base_code(uint32_t c) {
 //            most 8 bits so that every tile holds tens of keys per digit:
 //            each tile is sorted by digit in LDS and written as runs
 
-constexpr int L1_NT = 256;
+constexpr int L1_NT = 512;
 constexpr int MS_NT = 512;
 constexpr int L1_MAX_BITS = PK_TUNING_L1_BITS_MAX;  // level-1 digit width (direct scatter)
 constexpr int L1_DEFAULT_BITS = 8;  // wider digits keep more partial lines open
@@ -2626,16 +2626,16 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   const int l1max = tun.l1_bits > 0 && tun.l1_bits <= L1_MAX_BITS ? tun.l1_bits : L1_DEFAULT_BITS;
   const int d1 = btot < l1max ? btot : l1max;
   const uint64_t B1 = 1ull << d1;
-  // reads per block: up to 8 chunks of 256, fewer for small inputs so that
+  // reads per block: up to 8 chunks of L1_NT, fewer for small inputs so that
   // the grid still spreads over the CUs
   const int rpb = L1_NT * (tun.l1_chunks > 0 && tun.l1_chunks <= 64
                                ? tun.l1_chunks
                                : (int)std::min<uint64_t>(8, std::max<uint64_t>(1, R / (L1_NT * 1024ull))));
   const uint64_t nblk = (R + rpb - 1) / rpb;
-  // LDS staging: room for 256 records of the input's mean size, + 25 %
+  // LDS staging: room for L1_NT records of the input's mean size, + 25 %
   int sb;
   {
-    const double want = 256.0 * ((double)n / (double)R) * 1.25;
+    const double want = (double)L1_NT * ((double)n / (double)R) * 1.25;
     sb = (int)std::min<double>(L1_SB_MAX, std::max<double>(L1_SB_MIN, want));
     sb = (sb + 1023) & ~1023;
   }
