@@ -98,3 +98,20 @@ def test_fastq_bacterial_size(ctx, gen_mod):
     ctx.count_format(fq, 32, "fastq")
     k2, c2 = ctx.table_host()
     assert np.array_equal(k1, k2) and np.array_equal(c1, c2)
+
+
+def test_device_resident_formats(ctx, oracle_mod):
+    """Device-resident FASTQ / multi-line input (data_on_device) gives the
+    strict FASTA's table, and leaves the caller's buffer unchanged."""
+    import torch
+    fa = read_fasta("syn_mid.fa.gz")
+    ok, oc = oracle_mod.count(fa, 32)
+    for fmt, data in (("fastq", to_fastq(fa, b"\r\n")), ("fasta", to_multiline(fa, 61))):
+        _p(f"case: device-resident {fmt}")
+        host = np.frombuffer(data, dtype=np.uint8)
+        dev = torch.from_numpy(host.copy()).to("cuda:0")
+        torch.cuda.synchronize()
+        ctx.count_format(None, 32, fmt, device_ptr=dev.data_ptr(), nbytes=len(data))
+        keys, counts = ctx.table_host()
+        assert np.array_equal(keys, ok) and np.array_equal(counts, oc)
+        assert np.array_equal(dev.cpu().numpy(), host)
