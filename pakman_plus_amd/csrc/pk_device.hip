@@ -360,7 +360,7 @@ __global__ __launch_bounds__(NL_NT) void k_nl_write(const uint8_t* fa, uint64_t 
 // reads.  Line j of the input is [a, b): a = 0 or one past the (j-1)-th
 // '\n', b = the j-th '\n' or the end; a '\r' at b - 1 is dropped.  One
 // thread per line computes the line's output length, a scan places the
-// lines, and one wave per line copies it.
+// lines, and a byte-parallel pass (k_fmt_copy) moves the bytes.
 
 constexpr int FMT_FASTA = 1;
 constexpr int FMT_FASTQ = 2;
