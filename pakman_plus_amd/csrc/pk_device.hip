@@ -1341,7 +1341,7 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
       const int dpos = r2 - dbits;
       const int nbins = 1 << dbits;
       Key rk[PER];
-      uint32_t rc[PER];
+      uint32_t rc[PER], ri[PER];  // ri: index of the key inside its sub-bin
       ull ndk;  // distinct keys of the pass
       for (int i = threadIdx.x; i < nbins; i += BS_NT) hs[i] = 0;
       __syncthreads();
@@ -1350,7 +1350,7 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
         const int slot = threadIdx.x * PER + j;
         rc[j] = lds.hc[slot];
         rk[j] = rc[j] ? ht.key(slot) : Key(0);
-        if (rc[j]) atomicAdd(&hs[key_digit<Key>(rk[j], dpos, dbits)], 1u);
+        ri[j] = rc[j] ? atomicAdd(&hs[key_digit<Key>(rk[j], dpos, dbits)], 1u) : 0u;
       }
       __syncthreads();
       {
@@ -1380,7 +1380,7 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
 #pragma unroll
       for (int j = 0; j < PER; j++) {
         if (rc[j]) {
-          const uint32_t pos = atomicAdd(&hs[key_digit<Key>(rk[j], dpos, dbits)], 1u);
+          const uint32_t pos = hs[key_digit<Key>(rk[j], dpos, dbits)] + ri[j];
           ok[pos] = rk[j];
           oc[pos] = rc[j];
         }
