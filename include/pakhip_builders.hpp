@@ -7,7 +7,12 @@
 //       default_count_and_process).  Same constructor arguments, same
 //       return value (reads counted), same processor protocol
 //       (processor(begin, end) over pairs with ->first = k-mer word,
-//       ->second = int count), std::logic_error on failure.
+//       ->second = int count), std::logic_error on failure.  Also
+//       default_count_and_process_sort and concurrent_count_and_process_omp_v3
+//       (:1832-1930, the one the reference's unit test calls).  The other
+//       concurrent_* variants are left out: their results depend on the
+//       OpenMP thread count (per-thread byte or window ranges) and the
+//       omp_explicit / omp_implicit_* ones never call the processor.
 //   pakhip::count_kmers / pakhip::kmer_histogram / pakhip::macro_nodes
 //       the main path (perform_kmer_counting, src/kmer_counting.cpp:3351;
 //       construct_kmer_histogram, src/kmer_histogram.cpp:72;
@@ -97,6 +102,48 @@ class Hip_Kmer_Partial_Counter {
   // number of reads counted.
   template <typename processor_t>
   size_t default_count_and_process(processor_t& processor) {
+    std::vector<kmer_pair_t, alloc<kmer_pair_t>> pairs;
+    const size_t reads = count_pairs(pairs);
+    const size_t step = max_kmer_count_ ? max_kmer_count_ : 1;
+    if (pairs.size() <= step) {
+      processor(pairs.cbegin(), pairs.cend());
+    } else {
+      for (size_t b = 0; b < pairs.size(); b += step) {
+        const size_t e = b + step < pairs.size() ? b + step : pairs.size();
+        processor(pairs.cbegin() + b, pairs.cbegin() + e);
+      }
+    }
+    symbol_begin_ = symbol_end_;  // the reference's scan consumes the range
+    return reads;
+  }
+
+  // The reference's sibling counters produce the same table; on the device
+  // they are one implementation.
+  template <typename processor_t>
+  size_t default_count_and_process_sort(processor_t& processor) {
+    return default_count_and_process(processor);
+  }
+
+  // concurrent_count_and_process_omp_v3 (concurrent_kmer_counter.hpp:
+  // 1832-1930, the variant the reference's unit test drives): the same
+  // records, k-mers and read count as default_count_and_process on FASTA whose
+  // '>' bytes start records, handed to the processor in one call over the
+  // whole table (whatever max_kmer_count is), and the range is not consumed
+  // (the reference scans it with local pointers).
+  template <typename processor_t>
+  size_t concurrent_count_and_process_omp_v3(processor_t& processor) {
+    std::vector<kmer_pair_t, alloc<kmer_pair_t>> pairs;
+    const size_t reads = count_pairs(pairs);
+    processor(pairs.cbegin(), pairs.cend());
+    return reads;
+  }
+
+  pk_ctx* context() const { return ctx_.get(); }
+
+ private:
+  // the (k-mer, count) pairs of the counter's range, ascending; returns the
+  // reads counted
+  size_t count_pairs(std::vector<kmer_pair_t, alloc<kmer_pair_t>>& pairs) {
     const size_t n = (symbol_begin_ && symbol_end_ && symbol_end_ > symbol_begin_)
                          ? (size_t)(symbol_end_ - symbol_begin_)
                          : 0;
@@ -114,33 +161,13 @@ class Hip_Kmer_Partial_Counter {
     if (t.n)
       throw_on_error(ctx_.get(), pk_download_table(ctx_.get(), words.data(), counts.data()),
                      "pk_download_table");
-    std::vector<kmer_pair_t, alloc<kmer_pair_t>> pairs;
+    pairs.clear();
     pairs.reserve((size_t)t.n);
     for (size_t i = 0; i < (size_t)t.n; i++)
       pairs.emplace_back(kmer_word_traits<kmer_t>::from_words(&words[i * t.words]), counts[i]);
-    const size_t step = max_kmer_count_ ? max_kmer_count_ : 1;
-    if (pairs.size() <= step) {
-      processor(pairs.cbegin(), pairs.cend());
-    } else {
-      for (size_t b = 0; b < pairs.size(); b += step) {
-        const size_t e = b + step < pairs.size() ? b + step : pairs.size();
-        processor(pairs.cbegin() + b, pairs.cbegin() + e);
-      }
-    }
-    symbol_begin_ = symbol_end_;  // the reference's scan consumes the range
     return (size_t)reads;
   }
 
-  // The reference's sibling counters produce the same table; on the device
-  // they are one implementation.
-  template <typename processor_t>
-  size_t default_count_and_process_sort(processor_t& processor) {
-    return default_count_and_process(processor);
-  }
-
-  pk_ctx* context() const { return ctx_.get(); }
-
- private:
   size_t klen_;
   const char* symbol_begin_;
   const char* symbol_end_;

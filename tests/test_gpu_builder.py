@@ -160,6 +160,24 @@ def test_cpp_adapter_reference_shape(oracle_mod, tmp_path):
     assert r["xor"] == _digest(ok, oc)
 
 
+def test_cpp_adapter_omp_v3(oracle_mod, tmp_path):
+    """concurrent_count_and_process_omp_v3, as the reference's unit test calls
+    it (src/concurrent_kmer_counter_test.cpp:88-111): same reads and table as
+    the default counter, one processor call whatever max_kmer_count is, and
+    the range is not consumed (two calls agree)."""
+    _p("case: C++ adapter omp_v3")
+    r = _run_adapter("v3")
+    assert r["reads"] == 2 and r["freq"] == 138 and r["calls"] == 1
+    fa = read_fasta("syn_mid.fa.gz")
+    f = tmp_path / "mid.fa"
+    f.write_bytes(fa)
+    for klen, bits in ((32, 64), (64, 128)):
+        oreads, ok, oc = oracle_mod.builder_count(fa, klen, 100, bits)
+        r = _run_adapter("v3", f, klen, 100, bits, 1000)
+        assert r["reads"] == oreads and r["unique"] == ok.shape[0] and r["calls"] == 1
+        assert r["xor"] == _digest(ok, oc)
+
+
 @pytest.mark.parametrize("name,k,threshold,cov", [("syn_mid.fa.gz", 32, 20, 100), ("syn_rep.fa.gz", 32, 14, 30),
                                                   ("syn_mid.fa.gz", 64, 10, 50)])
 def test_pakgraph_nodes(oracle_mod, tmp_path, name, k, threshold, cov):
