@@ -2,6 +2,11 @@
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 CPU-baseline leg of bench.py, never by the pakman_plus_amd package.
+
+Besides the wrapper, normalize() restates the input-format rewrite of
+pk_count_format in Python; it is pinned by known answers and by round trips
+(strict FASTA unchanged; FASTQ and wrapped multi-line forms of the golden
+sets, with '\n' or '\r\n', rewrite to the original: tests/test_formats_cpu.py).
 """
 from __future__ import annotations
 
