@@ -116,12 +116,17 @@ uint64_t pk_ctx_device_bytes(const pk_ctx* ctx);
  *   l1_bits        level-1 radix digit width (1..PK_TUNING_L1_BITS_MAX; built-in 8)
  *   bucket_factor  instances per final bucket as a multiple of the LDS hash
  *                  table's key capacity, i.e. the inverse hash load target
- *                  (built-in 3.0) */
+ *                  (built-in 3.0)
+ *   ms_scatter     form of the further partition levels' scatter: 1 =
+ *                  persistent blocks (two per CU, consecutive tiles each),
+ *                  2 = one block per tile (built-in: persistent once every
+ *                  block gets >= 256 tiles) */
 #define PK_TUNING_L1_BITS_MAX 10
 typedef struct pk_tuning {
   int32_t l1_chunks;
   int32_t l1_bits;
   float bucket_factor;
+  int32_t ms_scatter;
 } pk_tuning;
 int pk_ctx_set_tuning(pk_ctx* ctx, const pk_tuning* t);
 

@@ -53,7 +53,7 @@ class _Timing(ctypes.Structure):
 class Tuning(ctypes.Structure):
     """Launch-shape choices of the counting kernels (pk_tuning); 0 = built-in."""
     _fields_ = [("l1_chunks", ctypes.c_int32), ("l1_bits", ctypes.c_int32),
-                ("bucket_factor", ctypes.c_float)]
+                ("bucket_factor", ctypes.c_float), ("ms_scatter", ctypes.c_int32)]
 
 
 class _TuneEntry(ctypes.Structure):
@@ -249,8 +249,9 @@ class Context:
     def device_bytes(self) -> int:
         return int(self._lib.pk_ctx_device_bytes(self._h))
 
-    def set_tuning(self, l1_chunks: int = 0, l1_bits: int = 0, bucket_factor: float = 0.0):
-        t = Tuning(l1_chunks, l1_bits, bucket_factor)
+    def set_tuning(self, l1_chunks: int = 0, l1_bits: int = 0, bucket_factor: float = 0.0,
+                   ms_scatter: int = 0):
+        t = Tuning(l1_chunks, l1_bits, bucket_factor, ms_scatter)
         self._check(self._lib.pk_ctx_set_tuning(self._h, ctypes.byref(t)))
 
     # -- perform_kmer_counting ------------------------------------------------
@@ -555,12 +556,13 @@ class Server:
 
     def set_tuning(self, entries: list[dict]):
         """entries: dicts with max_bytes, key_words, l1_chunks, l1_bits,
-        bucket_factor (the autotuner's table, pakman_plus_amd.tune)."""
+        bucket_factor, ms_scatter (the autotuner's table, pakman_plus_amd.tune)."""
         arr = (_TuneEntry * max(len(entries), 1))()
         for i, e in enumerate(entries):
             arr[i] = _TuneEntry(int(e["max_bytes"]), int(e.get("key_words", 0)),
                                 Tuning(int(e.get("l1_chunks", 0)), int(e.get("l1_bits", 0)),
-                                       float(e.get("bucket_factor", 0.0))))
+                                       float(e.get("bucket_factor", 0.0)),
+                                       int(e.get("ms_scatter", 0))))
         self._check(self._lib.pk_server_set_tuning(self._h, arr, len(entries)))
 
 

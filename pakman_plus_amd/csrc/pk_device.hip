@@ -1004,6 +1004,17 @@ __global__ void k_ms_boff(const ull* O, const ull* tp, int nb, int dbits, ull to
   if (g == n2) boff2[n2] = total;
 }
 
+// Tile geometry: bucket a (advanced from the previous tile's, tiles being
+// visited in order), first key b0 and key count m of tile g.
+__device__ __forceinline__ void ms_tile(const ull* boff, const ull* tp, ull g, int TQ, int& a, ull& t,
+                                        ull& b0, int& m) {
+  while (tp[a + 1] <= g) a++;
+  t = g - tp[a];
+  b0 = boff[a] + t * TQ;
+  const ull e = boff[a + 1];
+  m = (int)((b0 + TQ < e ? b0 + TQ : e) - b0);
+}
+
 template <class Key>
 __global__ __launch_bounds__(MS_NT) void k_ms_scatter(const Key* Sin, const ull* boff, const ull* tp,
                                                       int nb, int dpos, int dbits, const ull* O,
@@ -1054,6 +1065,88 @@ __global__ __launch_bounds__(MS_NT) void k_ms_scatter(const Key* Sin, const ull*
   }
   __syncthreads();
   for (int p = threadIdx.x; p < m; p += MS_NT) Sout[L.delta[L.D1[p]] + p] = L.T1[p];
+}
+
+// Persistent form for long levels: block i takes the tiles
+// [i * per, (i + 1) * per) in order and loads the next tile's keys and
+// output bases while it sorts the current one (at most 128 VGPRs: two blocks
+// of 8 waves per CU).  Consecutive tiles of a bucket write adjacent runs of
+// each sub-digit, so one CU completes the lines at their joins.
+template <class Key>
+__global__ __launch_bounds__(MS_NT, 4) void k_ms_scatter_p(const Key* Sin, const ull* boff, const ull* tp,
+                                                           int nb, int dpos, int dbits, const ull* O,
+                                                           Key* Sout) {
+  constexpr int Q = TileCfg<Key>::Q;
+  constexpr int TQ = MS_NT * TileCfg<Key>::Q;
+  __shared__ TileLds<Key, MS_NT> L;
+  const ull ntl = tp[nb];
+  const ull per = (ntl + gridDim.x - 1) / gridDim.x;
+  const ull g0 = (ull)blockIdx.x * per;
+  const ull g1 = g0 + per < ntl ? g0 + per : ntl;
+  if (g0 >= g1) return;
+  const int ns = 1 << dbits;
+  const int tid = (int)threadIdx.x;
+  // the current tile's keys (tid, tid + MS_NT, ...) and this thread's
+  // sub-digit base (tid < ns)
+  int a = tile_owner(tp, nb, g0);
+  ull t, b0;
+  int m;
+  ms_tile(boff, tp, g0, TQ, a, t, b0, m);
+  Key kr[Q];
+#pragma unroll
+  for (int j = 0; j < Q; j++) {
+    const int idx = tid + j * MS_NT;
+    kr[j] = idx < m ? Sin[b0 + idx] : Key(0);
+  }
+  ull bs = tid < ns ? O[ms_index(tp, a, tid, t, dbits)] : 0ull;
+  for (ull g = g0; g < g1; g++) {
+    Key kn[Q] = {};
+    ull bn = 0, tn = 0, b0n = 0;
+    int an = a, mn = 0;
+    if (g + 1 < g1) {
+      ms_tile(boff, tp, g + 1, TQ, an, tn, b0n, mn);
+#pragma unroll
+      for (int j = 0; j < Q; j++) {
+        const int idx = tid + j * MS_NT;
+        kn[j] = idx < mn ? Sin[b0n + idx] : Key(0);
+      }
+      if (tid < ns) bn = O[ms_index(tp, an, tid, tn, dbits)];
+    }
+    if (tid < ns) {
+      L.base[tid] = bs;
+      L.h[tid] = 0;
+    }
+    const int np = m > tid ? (m - tid - 1) / MS_NT + 1 : 0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < Q; j++)
+      if (j < np) atomicAdd(&L.h[key_digit<Key>(kr[j], dpos, dbits)], 1u);
+    __syncthreads();
+    const uint32_t v = tid < ns ? L.h[tid] : 0u;
+    const ull ex = block_excl_scan<MS_NT>(v);
+    if (tid < ns) {
+      L.tcur[tid] = (uint32_t)ex;
+      L.delta[tid] = (long long)L.base[tid] - (long long)ex;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+      if (j < np) {
+        const uint32_t d = key_digit<Key>(kr[j], dpos, dbits);
+        const uint32_t pos = atomicAdd(&L.tcur[d], 1u);
+        L.T1[pos] = kr[j];
+        L.D1[pos] = (uint8_t)d;
+      }
+    }
+    __syncthreads();
+    for (int p = tid; p < m; p += MS_NT) Sout[L.delta[L.D1[p]] + p] = L.T1[p];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < Q; j++) kr[j] = kn[j];
+    bs = bn;
+    a = an;
+    m = mn;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1121,6 +1214,8 @@ struct HashTable<uint64_t> {
   }
   // returns false when no free slot is found within HASH_PROBES probes, or
   // when a long chain finds the table already reported full (*stop)
+  // (a read before the CAS, as for u128 keys, was slower here: a new key
+  // then waits on two LDS round trips)
   __device__ __forceinline__ bool try_slot(uint32_t slot, uint64_t k) {
     const ull prev = atomicCAS(&hk[slot], empty, (ull)k);
     if (prev == empty || prev == (ull)k) {
@@ -1167,9 +1262,15 @@ struct HashTable<u128> {
     }
   }
   __device__ __forceinline__ uint32_t home(ull f) const { return (uint32_t)(f >> 40) & (T - 1); }
+  // a slot changes only once, from EMPTY to its fingerprint: a plain read
+  // finds a repeated key or steps over another one without an atomic, and
+  // only an empty slot is claimed by CAS (k = 64 bacterial: 2.73 -> 2.56 ms)
   __device__ __forceinline__ bool try_slot(uint32_t slot, ull f, u128 k) {
-    const ull prev = atomicCAS(&hf[slot], EMPTY, f);
-    if (prev == EMPTY) hkf[slot] = k;
+    ull prev = __hip_atomic_load(&hf[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (prev == EMPTY) {
+      prev = atomicCAS(&hf[slot], EMPTY, f);
+      if (prev == EMPTY) hkf[slot] = k;
+    }
     if (prev == EMPTY || prev == f) {
       atomicAdd(&hc[slot], 1u);
       return true;
@@ -2151,6 +2252,7 @@ enum BufId {
 
 struct pk_ctx {
   int device = 0;
+  int ncu = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   uint64_t budget = 0;
   uint64_t held = 0;
@@ -2716,8 +2818,18 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
     hipLaunchKernelGGL(k_ms_boff, dim3(grid_for(B2 + 1, 256)), dim3(256), 0, st, O2, tp, (int)B, db,
                        (ull)N, boff2);
     PK_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_scatter<Key>), dim3((unsigned)ntiles), dim3(MS_NT), 0, st,
-                       Sin, boff, tp, (int)B, dpos, db, O2, Sout);
+    // a long level runs on persistent blocks, two per CU (a tile buffer
+    // takes 74-78 KB of LDS), once each gets >= 256 tiles: measured faster
+    // at ~1400 tiles per block (k = 64, 2.9e9 instances), slower at ~60
+    // (k = 32, 2.5e8 instances) than one block per tile
+    const ull pgrid = 2ull * (ull)c->ncu;
+    const bool persistent = tun.ms_scatter ? tun.ms_scatter == 1 : ntiles >= 256 * pgrid;
+    if (persistent)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_scatter_p<Key>), dim3((unsigned)pgrid), dim3(MS_NT), 0, st,
+                         Sin, boff, tp, (int)B, dpos, db, O2, Sout);
+    else
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_scatter<Key>), dim3((unsigned)ntiles), dim3(MS_NT), 0, st,
+                         Sin, boff, tp, (int)B, dpos, db, O2, Sout);
     PK_HIP(c, hipGetLastError());
     std::swap(Sin, Sout);
     boff = boff2;
@@ -3040,6 +3152,9 @@ int pk_ctx_create(pk_ctx** out, int device, uint64_t hbm_budget_bytes) {
   pk_ctx* c = new pk_ctx();
   c->device = device;
   c->budget = hbm_budget_bytes;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+    c->ncu = ncu;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return PK_EHIP;
@@ -3081,7 +3196,7 @@ int pk_ctx_set_tuning(pk_ctx* c, const pk_tuning* t) {
     return PK_OK;
   }
   if (t->l1_chunks < 0 || t->l1_chunks > 64 || t->l1_bits < 0 || t->l1_bits > L1_MAX_BITS ||
-      !(t->bucket_factor >= 0.f && t->bucket_factor <= 64.f))
+      !(t->bucket_factor >= 0.f && t->bucket_factor <= 64.f) || t->ms_scatter < 0 || t->ms_scatter > 2)
     return c->fail(PK_EINVAL, "tuning values out of range");
   c->tuning = *t;
   return PK_OK;

@@ -53,7 +53,7 @@ def for_size(entries: list[dict], nbytes: int, k: int = 32) -> dict:
     words = 1 if k <= 32 else 2
     for e in sorted(entries, key=lambda e: e["max_bytes"]):
         if nbytes <= e["max_bytes"] and e.get("key_words", 0) in (0, words):
-            return {x: e[x] for x in ("l1_chunks", "l1_bits", "bucket_factor")}
+            return {x: e[x] for x in ("l1_chunks", "l1_bits", "bucket_factor", "ms_scatter") if x in e}
     return {}
 
 

@@ -220,6 +220,34 @@ def test_repeat_family_k64(ctx, oracle_mod, gen_mod):
     _check_nodes(ctx, oracle_mod, 64, kk, kc)
 
 
+@pytest.mark.parametrize("case", ["small_k32", "repeats_k64", "skewed_k32"])
+def test_persistent_partition_scatter(ctx, oracle_mod, gen_mod, case):
+    """The persistent form of the further partition levels (each block walks
+    many consecutive tiles, crossing bucket ends and empty buckets), forced
+    at sizes where every block gets several tiles, against the oracle."""
+    _p(f"case: persistent partition scatter {case}")
+    if case == "small_k32":
+        fa, k = gen_mod.make_fasta(gen_mod.config_small()), 32
+    elif case == "repeats_k64":
+        fa, k = gen_mod.make_fasta(gen_mod.config_large(scale=0.02)), 64
+    else:
+        # 3 in 8 reads one repeated k-mer, the rest random over A, C, G: one
+        # huge bucket, and every level-1 bucket whose digit holds a T empty
+        rng = np.random.default_rng(11)
+        bases = np.frombuffer(b"ACG", dtype=np.uint8)
+        parts = [b">a%d\n" % i + b"A" * 100 + b"\n" for i in range(60000)]
+        parts += [b">r%d\n" % i + bases[rng.integers(0, 3, 100)].tobytes() + b"\n" for i in range(100000)]
+        fa, k = b"".join(parts), 32
+    try:
+        ctx.set_tuning(ms_scatter=1, bucket_factor=1.5)
+        _check_table(ctx, oracle_mod, fa, k)
+        # more than two tiles (8192 one-word or 4096 two-word keys) for each
+        # of the 512 blocks of a 256-CU device
+        assert ctx.timing()["n_instances"] > 2 * 512 * (4096 if k > 32 else 8192)
+    finally:
+        ctx.set_tuning()
+
+
 def test_bacterial_full_size_properties(ctx, gen_mod):
     """4.6 Mbp / 80x / k=32 at full size: size-independent properties (sum of
     counts = number of k-mer instances, strictly ascending keys, counts >= 1,
@@ -290,7 +318,9 @@ def test_rank_share(ctx, oracle_mod, k, P):
                                     dict(l1_chunks=2, l1_bits=10, bucket_factor=2.2),
                                     # buckets far beyond a hash table: every
                                     # bucket overflows its first passes
-                                    dict(l1_chunks=1, l1_bits=9, bucket_factor=24.0)])
+                                    dict(l1_chunks=1, l1_bits=9, bucket_factor=24.0),
+                                    # both forms of the further levels' scatter
+                                    dict(ms_scatter=1), dict(ms_scatter=2, bucket_factor=1.5)])
 def test_tuning_does_not_change_results(ctx, oracle_mod, tuning):
     """Launch-shape choices of the autotuner leave every output unchanged."""
     _p(f"case: tuning {tuning}")
@@ -310,7 +340,8 @@ def test_tuning_does_not_change_results(ctx, oracle_mod, tuning):
 
 def test_tuning_out_of_range(ctx):
     from pakman_plus_amd import api
-    for bad in (dict(l1_bits=11), dict(l1_chunks=65), dict(bucket_factor=65.0)):
+    for bad in (dict(l1_bits=11), dict(l1_chunks=65), dict(bucket_factor=65.0), dict(ms_scatter=3),
+                dict(ms_scatter=-1)):
         with pytest.raises(api.PakHipError) as e:
             ctx.set_tuning(**bad)
         assert e.value.code == api.PK_EINVAL

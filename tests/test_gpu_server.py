@@ -168,7 +168,8 @@ def test_server_tuning_entries_checked():
     with api.Server(0, n_slots=1) as srv:
         good = {"max_bytes": 1 << 30, "key_words": 1, "l1_chunks": 2, "l1_bits": 10, "bucket_factor": 2.2}
         srv.set_tuning([good])
-        for bad in ({"l1_bits": 11}, {"l1_chunks": 65}, {"key_words": 3}, {"bucket_factor": float("nan")}):
+        for bad in ({"l1_bits": 11}, {"l1_chunks": 65}, {"key_words": 3}, {"bucket_factor": float("nan")},
+                    {"ms_scatter": 3}):
             with pytest.raises(api.PakHipError) as e:
                 srv.set_tuning([good, dict(good, **bad)])
             assert e.value.code == api.PK_EINVAL
