@@ -627,7 +627,7 @@ __device__ __forceinline__ uint32_t base_code(uint32_t c) {
 //            most 8 bits so that every tile holds tens of keys per digit:
 //            each tile is sorted by digit in LDS and written as runs
 
-constexpr int L1_NT = 512;
+constexpr int L1_NT = 512;  // level-1 histogram blocks
 constexpr int MS_NT = 512;
 constexpr int L1_MAX_BITS = PK_TUNING_L1_BITS_MAX;  // level-1 digit width (direct scatter)
 constexpr int L1_DEFAULT_BITS = 8;  // wider digits keep more partial lines open
@@ -653,16 +653,29 @@ struct TileLds {
 };
 
 // Level-1 producer.  A block owns rpb consecutive reads and walks them in
-// chunks: a chunk is the longest run of <= L1_NT consecutive reads whose
-// bytes fit in the LDS staging buffer; thread t then walks read t of the
+// chunks: a chunk is the longest run of <= NT consecutive reads (NT threads
+// per block) whose bytes fit in the LDS staging buffer; thread t then walks
+// read t of the
 // chunk from LDS with the rolling update (a read too long for the buffer is
 // walked from global memory, alone in its chunk).  Records are in file
 // order, so a chunk's bytes are one contiguous range.  The buffer holds the
 // base code of every byte, not the byte: bits 0-1 of the code in plane A
 // (32 bytes per word) and, on the main path, bit 2 in plane B (32 bytes per
 // word), 3/8 of the raw size.  Its size (sb bytes, in dynamic LDS) is chosen
-// per input so that 256 typical records fit.
+// per input so that NT typical records fit.
 constexpr int L1_SB_MIN = 8192, L1_SB_MAX = 98304;  // staged bytes per chunk
+
+// Level-1 scatter blocks: 1024 threads for two-word keys on the main path
+// (two blocks per CU, <= 60 KB of LDS and <= 64 VGPRs each; k = 64
+// bacterial 1.50 -> 1.34 ms), 512 otherwise (one-word keys measured
+// 0.02-0.04 ms faster at 512; the rank-share and builder walks need more
+// registers).
+template <class Key, int ENC, bool OWN>
+struct L1Scatter {
+  static constexpr bool wide = sizeof(Key) == 16 && ENC == ENC_MAIN && !OWN;
+  static constexpr int NT = wide ? 1024 : 512;
+  static constexpr int SB_MAX = wide ? 131072 : L1_SB_MAX;
+};
 
 struct ChunkLds {
   ull base;  // first staged byte (32-byte aligned)
@@ -683,16 +696,16 @@ __host__ __device__ constexpr uint32_t l1_lds_bytes(int sb, int nd, int dig_byte
 }
 
 // Sets up the chunk starting at read c0 (< r1) and stages its codes.
-template <int ENC>
+template <int ENC, int NT>
 __device__ __forceinline__ void stage_chunk(ChunkLds& C, Stage sg, const uint8_t* fa, uint64_t n,
-                                            const ull* rstart, const uint32_t* rlen, uint64_t c0, uint64_t r1,
-                                            int NT) {
+                                            const ull* rstart, const uint32_t* rlen, uint64_t c0,
+                                            uint64_t r1) {
   const uint64_t r = c0 + threadIdx.x;
   const ull first = rstart[c0] & ~31ull;
   int fits = 0;
   if (r < r1 && (int)threadIdx.x < NT) fits = rstart[r] + rlen[r] - first <= (ull)sg.sb;
   // records ascend in the file: the reads that fit are a prefix
-  const int m = (int)block_sum<L1_NT>((ull)fits);
+  const int m = (int)block_sum<NT>((ull)fits);
   if (threadIdx.x == 0) {
     C.base = first;
     C.m = m > 0 ? m : 1;
@@ -869,7 +882,7 @@ __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n
   const uint64_t r1 = r0 + rpb < R ? r0 + rpb : R;
   const int bpos = KB - d1;
   for (uint64_t c0 = r0; c0 < r1;) {
-    stage_chunk<ENC>(C, sg, fa, n, rstart, rlen, c0, r1, L1_NT);
+    stage_chunk<ENC, L1_NT>(C, sg, fa, n, rstart, rlen, c0, r1);
     const int m = C.m;
     if ((int)threadIdx.x < m) {
       const uint64_t r = c0 + threadIdx.x;
@@ -885,17 +898,20 @@ __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n
   for (int i = threadIdx.x; i < nb; i += L1_NT) H[(ull)i * gridDim.x + blockIdx.x] = h[i];
 }
 
-// Level-1 scatter fused with the parse: the same chunked walk as k_l1_hist;
-// every key takes the next slot of its digit from the block's LDS counter
-// (atomicAdd's return value) and is stored at once at the block's base for
-// that digit (scanned histogram O, same layout as H).  No tile buffer and no
-// barrier inside the walk: the order of keys inside a bucket is arbitrary,
-// which the bucket count does not depend on.
+// Level-1 scatter fused with the parse: the same chunked walk as k_l1_hist
+// over the same reads per block, in blocks of L1Scatter::NT threads; every key
+// takes the next slot of its digit from the block's LDS counter (atomicAdd's
+// return value) and is stored at once at the block's base for that digit
+// (scanned histogram O, same layout as H).  No tile buffer and no barrier
+// inside the walk: the order of keys inside a bucket is arbitrary, which the
+// bucket count does not depend on.  Each block keeps a partly written line
+// open per digit, so fewer, larger blocks per CU write fewer partial lines.
 template <class Key, int ENC, bool OWN>
-__global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_t n,
+__global__ __launch_bounds__((L1Scatter<Key, ENC, OWN>::NT)) void k_l1_scatter(const uint8_t* fa, uint64_t n,
                                                       const ull* rstart, const uint32_t* rlen,
                                                       uint64_t R, int K, int KB, Key mask, int d1,
                                                       int rpb, int sb, OwnCfg cf, const ull* O, Key* S1) {
+  constexpr int NT = L1Scatter<Key, ENC, OWN>::NT;
   __shared__ ChunkLds C;
   // dynamic LDS: plane A, 2^d1 output cursors, plane B, 2^d1 ranks
   extern __shared__ ull l1_dyn[];
@@ -903,7 +919,7 @@ __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_
   ull* base = l1_dyn + sb / 32;
   const Stage sg{l1_dyn, reinterpret_cast<uint32_t*>(base + nb), sb};
   uint32_t* h = sg.B + sb / 32;
-  for (int i = threadIdx.x; i < nb; i += L1_NT) {
+  for (int i = threadIdx.x; i < nb; i += NT) {
     base[i] = O[(ull)i * gridDim.x + blockIdx.x];
     h[i] = 0;
   }
@@ -912,7 +928,7 @@ __global__ __launch_bounds__(L1_NT) void k_l1_scatter(const uint8_t* fa, uint64_
   const int bpos = KB - d1;
   // stage_chunk's barriers order the set-up above before the first walk
   for (uint64_t c0 = r0; c0 < r1;) {
-    stage_chunk<ENC>(C, sg, fa, n, rstart, rlen, c0, r1, L1_NT);
+    stage_chunk<ENC, NT>(C, sg, fa, n, rstart, rlen, c0, r1);
     const int m = C.m;
     if ((int)threadIdx.x < m) {
       const uint64_t r = c0 + threadIdx.x;
@@ -2725,8 +2741,19 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   const pk_tuning& tun = c->tuning;
   const float bf = tun.bucket_factor > 0.f ? tun.bucket_factor : 3.0f;
   const int btot = choose_bits(OWN ? (N + cf.P - 1) / cf.P : N, KB / 2, CAP, bf);
-  const int l1max = tun.l1_bits > 0 && tun.l1_bits <= L1_MAX_BITS ? tun.l1_bits : L1_DEFAULT_BITS;
-  const int d1 = btot < l1max ? btot : l1max;
+  int d1;
+  if (tun.l1_bits > 0 && tun.l1_bits <= L1_MAX_BITS) {
+    d1 = btot < tun.l1_bits ? btot : tun.l1_bits;
+  } else if (btot > L1_DEFAULT_BITS + MS_MAX_BITS) {
+    // three or more levels anyway: the narrowest level-1 digit (>= 5 bits)
+    // that adds no level, since every level-1 digit keeps a partly written
+    // line open per block (k = 64, 2.9e9 instances, 2^21 buckets: level-1
+    // scatter 33.8 ms at 8 bits, 13.8 ms at 5)
+    const int further = (btot - L1_DEFAULT_BITS + MS_MAX_BITS - 1) / MS_MAX_BITS;
+    d1 = std::max(5, btot - further * MS_MAX_BITS);
+  } else {
+    d1 = btot < L1_DEFAULT_BITS ? btot : L1_DEFAULT_BITS;
+  }
   const uint64_t B1 = 1ull << d1;
   // reads per block: up to 8 chunks of L1_NT, fewer for small inputs so that
   // the grid still spreads over the CUs
@@ -2734,13 +2761,16 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
                                ? tun.l1_chunks
                                : (int)std::min<uint64_t>(8, std::max<uint64_t>(1, R / (L1_NT * 1024ull))));
   const uint64_t nblk = (R + rpb - 1) / rpb;
-  // LDS staging: room for L1_NT records of the input's mean size, + 25 %
-  int sb;
-  {
-    const double want = (double)L1_NT * ((double)n / (double)R) * 1.25;
-    sb = (int)std::min<double>(L1_SB_MAX, std::max<double>(L1_SB_MIN, want));
-    sb = (sb + 1023) & ~1023;
-  }
+  // LDS staging: room for one record per thread of the input's mean size,
+  // + 25 %, for each of the two level-1 kernels
+  auto staging = [&](int nt, int cap) {
+    const double want = (double)nt * ((double)n / (double)R) * 1.25;
+    const int b = (int)std::min<double>(cap, std::max<double>(L1_SB_MIN, want));
+    return (b + 1023) & ~1023;
+  };
+  const int sb = staging(L1_NT, L1_SB_MAX);
+  constexpr int snt = L1Scatter<Key, ENC, OWN>::NT;
+  const int sbs = staging(snt, L1Scatter<Key, ENC, OWN>::SB_MAX);
   ull *boff, *O;
   uint32_t* H;
   Key* S2;
@@ -2776,8 +2806,8 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
                      boff);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[4], st));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key, ENC, OWN>), dim3((unsigned)nblk), dim3(L1_NT),
-                     l1_lds_bytes(sb, (int)B1, 12), st, fa, n, rstart, rlen, R, K, KB, mask, d1, rpb, sb, cf,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key, ENC, OWN>), dim3((unsigned)nblk), dim3(snt),
+                     l1_lds_bytes(sbs, (int)B1, 12), st, fa, n, rstart, rlen, R, K, KB, mask, d1, rpb, sbs, cf,
                      O, S);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[16], st));
