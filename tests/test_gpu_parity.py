@@ -320,7 +320,10 @@ def test_rank_share(ctx, oracle_mod, k, P):
                                     # bucket overflows its first passes
                                     dict(l1_chunks=1, l1_bits=9, bucket_factor=24.0),
                                     # both forms of the further levels' scatter
-                                    dict(ms_scatter=1), dict(ms_scatter=2, bucket_factor=1.5)])
+                                    dict(ms_scatter=1), dict(ms_scatter=2, bucket_factor=1.5),
+                                    # tiny buckets: three partition levels with the
+                                    # built-in level-1 width
+                                    dict(bucket_factor=0.001)])
 def test_tuning_does_not_change_results(ctx, oracle_mod, tuning):
     """Launch-shape choices of the autotuner leave every output unchanged."""
     _p(f"case: tuning {tuning}")
