@@ -60,15 +60,20 @@ def for_size(entries: list[dict], nbytes: int, k: int = 32) -> dict:
     return {}
 
 
-def tune(path: str = DEFAULT_PATH, reps: int = 5, ks=(32, 64), t: int = 21) -> dict:
+def tune(path: str = DEFAULT_PATH, reps: int = 5, ks=(32, 64), t: int = 21,
+         keep_from: str | None = DEFAULT_PATH) -> dict:
     """Tunes every size bucket for one-word keys (k = 32) and two-word keys
-    (k = 64): the hash tables of the two hold different numbers of keys."""
+    (k = 64): the hash tables of the two hold different numbers of keys.
+    Entries for key widths not tuned now are kept from keep_from."""
     entries = []
     for k in ks:
         entries += _tune_k(reps, k, t)
+    tuned = {1 if k <= 32 else 2 for k in ks}
+    kept = [e for e in load(keep_from) if e.get("key_words") not in tuned] if keep_from else []
     import torch
-    out = {"device": torch.cuda.get_device_name(0), "ks": list(ks), "t": t, "reps": reps,
-           "entries": entries}
+    out = {"device": torch.cuda.get_device_name(0),
+           "ks": sorted(set(ks) | {e["k"] for e in kept if "k" in e}), "t": t, "reps": reps,
+           "entries": sorted(kept + entries, key=lambda e: (e.get("key_words", 0), e["max_bytes"]))}
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
     return out
@@ -129,5 +134,6 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_PATH)
     ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--ks", default="32,64", help="k values to tune (one per key width)")
     a = ap.parse_args()
-    tune(a.out, a.reps)
+    tune(a.out, a.reps, tuple(int(x) for x in a.ks.split(",")))
