@@ -1373,7 +1373,6 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
   constexpr int HB = sizeof(Key) == 8 ? 16 : 8;
   __shared__ HashLds<Key> lds;
   __shared__ uint32_t hs[SUB_BINS];
-  __shared__ uint32_t he[SUB_BINS];
   __shared__ int over;
 
   const uint64_t task = blockIdx.x;
@@ -1487,7 +1486,6 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
           if (bb < nbins) {
             hs[bb] = (uint32_t)ex;
             ex += v[i];
-            he[bb] = (uint32_t)ex;
           }
         }
       }
@@ -1507,8 +1505,10 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
       for (uint32_t i = threadIdx.x; i < nd; i += BS_NT) {
         const Key v = ok[i];
         const int d = (int)key_digit<Key>(v, dpos, dbits);
-        const int a = d ? (int)he[d - 1] : 0;
-        const int e = (int)he[d];
+        // the sub-bin's slots: from its start to the next one's (no end
+        // array, so that six tables fit a CU's LDS)
+        const int a = (int)hs[d];
+        const int e = d + 1 < nbins ? (int)hs[d + 1] : (int)nd;
         int rank = 0;
         for (int j = a; j < e; j++) rank += ok[j] < v;
         R[start + out + a + rank] = v;
