@@ -1761,8 +1761,11 @@ __global__ __launch_bounds__(CUT_NT) void k_kept_write(const Key* TK, const int3
 // lists.  Range g covers node keys [s_g, s_{g+1}) with s_g = pred(KK[g*PC]).
 
 constexpr int MN_NT = 512;
-constexpr int MN_PC = 1024;    // pred entries per range
-constexpr int MN_ECAP = 3072;  // LDS capacity (entries of all five lists)
+// 768 pred entries per range keep a u64 block at 36 KB of LDS, four blocks
+// of 512 threads per CU (1024 / 3072 allowed three: 0.88 -> 0.82 ms for the
+// bacterial nodes)
+constexpr int MN_PC = 768;     // pred entries per range
+constexpr int MN_ECAP = 2304;  // LDS capacity (entries of all five lists)
 
 struct MnPlan {
   ull lo[5];
