@@ -639,16 +639,15 @@ struct TileCfg {
 };
 
 // LDS of a tile split by digit: a histogram and its exclusive scan, a
-// counting-sort scatter of the keys into T1 (digit of every sorted slot in
-// D1), then coalesced stores: sorted slot p of digit d goes to
-// out[base[d] - toff[d] + p] (delta[d] = base[d] - toff[d]).
+// counting-sort scatter of the keys into T1, then coalesced stores: sorted
+// slot p, of digit d (recomputed from the key), goes to out[base[d] -
+// toff[d] + p] (delta[d] = base[d] - toff[d]; base[d] stays in the register
+// of thread d).  No digit array: three tiles fit a CU's LDS.
 template <class Key, int NT>
 struct TileLds {
   Key T1[NT * TileCfg<Key>::Q];
-  uint8_t D1[NT * TileCfg<Key>::Q];
   uint32_t h[256];
   uint32_t tcur[256];
-  ull base[256];
   long long delta[256];
 };
 
@@ -1046,10 +1045,9 @@ __global__ __launch_bounds__(MS_NT) void k_ms_scatter(const Key* Sin, const ull*
   const ull b1 = b0 + TQ < boff[a + 1] ? b0 + TQ : boff[a + 1];
   const int m = (int)(b1 - b0);
   const int ns = 1 << dbits;
-  for (int i = threadIdx.x; i < ns; i += MS_NT) {
-    L.base[i] = O[ms_index(tp, a, i, t, dbits)];
-    L.h[i] = 0;
-  }
+  // sub-digit tid's output base (ns <= 256 <= MS_NT)
+  const ull bse = (int)threadIdx.x < ns ? O[ms_index(tp, a, threadIdx.x, t, dbits)] : 0ull;
+  if ((int)threadIdx.x < ns) L.h[threadIdx.x] = 0;
   // keys tid, tid + MS_NT, ... of the tile in registers (coalesced loads)
   Key kr[Q];
 #pragma unroll
@@ -1067,7 +1065,7 @@ __global__ __launch_bounds__(MS_NT) void k_ms_scatter(const Key* Sin, const ull*
   const ull ex = block_excl_scan<MS_NT>(v);
   if ((int)threadIdx.x < ns) {
     L.tcur[threadIdx.x] = (uint32_t)ex;
-    L.delta[threadIdx.x] = (long long)L.base[threadIdx.x] - (long long)ex;
+    L.delta[threadIdx.x] = (long long)bse - (long long)ex;
   }
   __syncthreads();
 #pragma unroll
@@ -1076,11 +1074,13 @@ __global__ __launch_bounds__(MS_NT) void k_ms_scatter(const Key* Sin, const ull*
       const uint32_t d = key_digit<Key>(kr[j], dpos, dbits);
       const uint32_t pos = atomicAdd(&L.tcur[d], 1u);
       L.T1[pos] = kr[j];
-      L.D1[pos] = (uint8_t)d;
     }
   }
   __syncthreads();
-  for (int p = threadIdx.x; p < m; p += MS_NT) Sout[L.delta[L.D1[p]] + p] = L.T1[p];
+  for (int p = threadIdx.x; p < m; p += MS_NT) {
+    const Key v = L.T1[p];
+    Sout[L.delta[key_digit<Key>(v, dpos, dbits)] + p] = v;
+  }
 }
 
 // Persistent form for long levels: block i takes the tiles
@@ -1128,10 +1128,7 @@ __global__ __launch_bounds__(MS_NT, 4) void k_ms_scatter_p(const Key* Sin, const
       }
       if (tid < ns) bn = O[ms_index(tp, an, tid, tn, dbits)];
     }
-    if (tid < ns) {
-      L.base[tid] = bs;
-      L.h[tid] = 0;
-    }
+    if (tid < ns) L.h[tid] = 0;
     const int np = m > tid ? (m - tid - 1) / MS_NT + 1 : 0;
     __syncthreads();
 #pragma unroll
@@ -1142,7 +1139,7 @@ __global__ __launch_bounds__(MS_NT, 4) void k_ms_scatter_p(const Key* Sin, const
     const ull ex = block_excl_scan<MS_NT>(v);
     if (tid < ns) {
       L.tcur[tid] = (uint32_t)ex;
-      L.delta[tid] = (long long)L.base[tid] - (long long)ex;
+      L.delta[tid] = (long long)bs - (long long)ex;
     }
     __syncthreads();
 #pragma unroll
@@ -1151,11 +1148,13 @@ __global__ __launch_bounds__(MS_NT, 4) void k_ms_scatter_p(const Key* Sin, const
         const uint32_t d = key_digit<Key>(kr[j], dpos, dbits);
         const uint32_t pos = atomicAdd(&L.tcur[d], 1u);
         L.T1[pos] = kr[j];
-        L.D1[pos] = (uint8_t)d;
       }
     }
     __syncthreads();
-    for (int p = tid; p < m; p += MS_NT) Sout[L.delta[L.D1[p]] + p] = L.T1[p];
+    for (int p = tid; p < m; p += MS_NT) {
+      const Key v = L.T1[p];
+      Sout[L.delta[key_digit<Key>(v, dpos, dbits)] + p] = v;
+    }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < Q; j++) kr[j] = kn[j];
