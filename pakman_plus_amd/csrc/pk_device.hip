@@ -642,7 +642,7 @@ struct TileCfg {
 // counting-sort scatter of the keys into T1, then coalesced stores: sorted
 // slot p, of digit d (recomputed from the key), goes to out[base[d] -
 // toff[d] + p] (delta[d] = base[d] - toff[d]; base[d] stays in the register
-// of thread d).  No digit array: three tiles fit a CU's LDS.
+// of thread d): about 68 KB for 8192 one-word or 4096 two-word keys.
 template <class Key, int NT>
 struct TileLds {
   Key T1[NT * TileCfg<Key>::Q];
@@ -654,9 +654,8 @@ struct TileLds {
 // Level-1 producer.  A block owns rpb consecutive reads and walks them in
 // chunks: a chunk is the longest run of <= NT consecutive reads (NT threads
 // per block) whose bytes fit in the LDS staging buffer; thread t then walks
-// read t of the
-// chunk from LDS with the rolling update (a read too long for the buffer is
-// walked from global memory, alone in its chunk).  Records are in file
+// read t of the chunk from LDS with the rolling update (a read too long for
+// the buffer is walked from global memory, alone in its chunk).  Records are in file
 // order, so a chunk's bytes are one contiguous range.  The buffer holds the
 // base code of every byte, not the byte: bits 0-1 of the code in plane A
 // (32 bytes per word) and, on the main path, bit 2 in plane B (32 bytes per
@@ -2851,7 +2850,7 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
                        (ull)N, boff2);
     PK_HIP(c, hipGetLastError());
     // a long level runs on persistent blocks, two per CU (a tile buffer
-    // takes 74-78 KB of LDS), once each gets >= 256 tiles: measured faster
+    // takes about 68 KB of LDS), once each gets >= 256 tiles: measured faster
     // at ~1400 tiles per block (k = 64, 2.9e9 instances), slower at ~60
     // (k = 32, 2.5e8 instances) than one block per tile
     const ull pgrid = 2ull * (ull)c->ncu;
