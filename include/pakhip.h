@@ -113,7 +113,9 @@ uint64_t pk_ctx_device_bytes(const pk_ctx* ctx);
  * them).  Zero fields keep the built-in choice.
  *   l1_chunks      reads per level-1 partition block, in chunks of 512 (1..64;
  *                  built-in: 1..8 by input size)
- *   l1_bits        level-1 radix digit width (1..PK_TUNING_L1_BITS_MAX; built-in 8)
+ *   l1_bits        level-1 radix digit width (1..PK_TUNING_L1_BITS_MAX; built-in 8,
+ *                  or, when the bucket count needs three or more partition
+ *                  levels, the narrowest width >= 5 that adds no level)
  *   bucket_factor  instances per final bucket as a multiple of the LDS hash
  *                  table's key capacity, i.e. the inverse hash load target
  *                  (built-in 3.0)
