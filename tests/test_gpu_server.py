@@ -106,6 +106,28 @@ def test_batch_equals_alone_and_oracle(oracle_mod, gen_mod):
         assert e.value.code == api.PK_EINVAL
 
 
+def test_batch_with_forced_tuning_equals_alone(oracle_mod, gen_mod):
+    """A tuning table that forces the persistent partition scatter and tiny
+    buckets (three partition levels, built-in level-1 width) leaves every
+    job's results equal to its plain run and the oracle, with four jobs on
+    the GPU at once."""
+    _p("case: server batch with forced tuning, 4 slots")
+    from pakman_plus_amd import api
+    jobs = _mixed_jobs(gen_mod)[:8]
+    with api.Context(0) as ctx:
+        alone = [_alone(ctx, j) for j in jobs]
+    table = [{"max_bytes": 1 << 40, "key_words": w, "l1_chunks": 1, "l1_bits": 0,
+              "bucket_factor": 0.005, "ms_scatter": 1} for w in (1, 2)]
+    with api.Server(0, n_slots=4) as srv:
+        srv.set_tuning(table)
+        tickets = srv.submit_batch(jobs)
+        results = [srv.wait(tk) for tk in tickets]
+        for job, res, want in zip(jobs, results, alone):
+            _compare(srv, res, want, oracle_mod, job)
+        for tk in tickets:
+            srv.release(tk)
+
+
 def test_served_stream_with_device_inputs(oracle_mod, gen_mod):
     """Jobs arrive from another thread while others run (device-resident
     inputs); each result equals the job alone."""
