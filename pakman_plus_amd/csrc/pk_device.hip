@@ -1733,12 +1733,19 @@ __global__ __launch_bounds__(CUT_NT) void k_kept_write(const Key* TK, const int3
   const int thr = *mb;
   const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER + (uint64_t)threadIdx.x * 4;
   int32_t cv[4];
+  if (base + 4 <= U) {
+    const int4 q = *reinterpret_cast<const int4*>(TC + base);
+    cv[0] = q.x;
+    cv[1] = q.y;
+    cv[2] = q.z;
+    cv[3] = q.w;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; i++) cv[i] = base + i < U ? TC[base + i] : 0;
+  }
   ull c = 0;
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    cv[i] = base + i < U ? TC[base + i] : 0;
-    c += (base + i < U) && cv[i] >= thr;
-  }
+  for (int i = 0; i < 4; i++) c += (base + i < U) && cv[i] >= thr;
   ull o = block_excl_scan<CUT_NT>(c) + boffs[blockIdx.x];
 #pragma unroll
   for (int i = 0; i < 4; i++) {
