@@ -2252,7 +2252,12 @@ __global__ void k_mn_compact(const MnRec* NT, const MnPlan* plan, const ull* ng,
   const uint64_t g = blockIdx.x;
   if (g >= G) return;
   const ull b = plan[g].base, o = noff[g], c = ng[g];
-  for (ull i = threadIdx.x; i < c; i += blockDim.x) out[o + i] = NT[b + i];
+  // a record is three 16-byte words (both arrays start 256-byte aligned):
+  // copy the range as words, one per lane
+  static_assert(sizeof(MnRec) == 48, "MnRec is three 16-byte words");
+  const uint4* src = reinterpret_cast<const uint4*>(NT + b);
+  uint4* dst = reinterpret_cast<uint4*>(out + o);
+  for (ull i = threadIdx.x; i < 3 * c; i += blockDim.x) dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------------------
