@@ -68,15 +68,22 @@ def tune(path: str = DEFAULT_PATH, reps: int = 5, ks=(32, 64), t: int = 21,
     entries = []
     for k in ks:
         entries += _tune_k(reps, k, t)
-    tuned = {1 if k <= 32 else 2 for k in ks}
-    kept = [e for e in load(keep_from) if e.get("key_words") not in tuned] if keep_from else []
+    ks_all, merged = merge(entries, ks, load(keep_from) if keep_from else [])
     import torch
-    out = {"device": torch.cuda.get_device_name(0),
-           "ks": sorted(set(ks) | {e["k"] for e in kept if "k" in e}), "t": t, "reps": reps,
-           "entries": sorted(kept + entries, key=lambda e: (e.get("key_words", 0), e["max_bytes"]))}
+    out = {"device": torch.cuda.get_device_name(0), "ks": ks_all, "t": t, "reps": reps,
+           "entries": merged}
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
     return out
+
+
+def merge(entries: list[dict], ks, previous: list[dict]) -> tuple[list[int], list[dict]]:
+    """New entries for the key widths of ks, plus the previous table's
+    entries for the other widths: (k values, entries by width and size)."""
+    tuned = {1 if k <= 32 else 2 for k in ks}
+    kept = [e for e in previous if e.get("key_words") not in tuned]
+    ks_all = sorted(set(ks) | {e["k"] for e in kept if "k" in e})
+    return ks_all, sorted(kept + entries, key=lambda e: (e.get("key_words", 0), e["max_bytes"]))
 
 
 def _tune_k(reps: int, k: int, t: int) -> list[dict]:

@@ -48,3 +48,19 @@ def test_bench_fastq_writer_matches_test_writer():
     from golden_util import read_fasta
     fa = read_fasta("syn_tiny.fa.gz")
     assert bench.fastq_crlf(fa) == to_fastq(fa, b"\r\n")
+
+
+def test_tuning_merge_keeps_other_key_width():
+    """Re-tuning one key width replaces its entries and keeps the other's."""
+    from pakman_plus_amd import tune
+    old = [{"max_bytes": 100, "key_words": 1, "k": 32, "bucket_factor": 3.0},
+           {"max_bytes": 1000, "key_words": 1, "k": 32, "bucket_factor": 5.0},
+           {"max_bytes": 100, "key_words": 2, "k": 64, "bucket_factor": 1.5}]
+    new = [{"max_bytes": 1000, "key_words": 2, "k": 64, "bucket_factor": 2.2},
+           {"max_bytes": 100, "key_words": 2, "k": 64, "bucket_factor": 9.0}]
+    ks, merged = tune.merge(new, (64,), old)
+    assert ks == [32, 64]
+    assert [(e["key_words"], e["max_bytes"], e["bucket_factor"]) for e in merged] == [
+        (1, 100, 3.0), (1, 1000, 5.0), (2, 100, 9.0), (2, 1000, 2.2)]
+    assert tune.for_size(merged, 50, 64)["bucket_factor"] == 9.0
+    assert tune.for_size(merged, 500, 32)["bucket_factor"] == 5.0
