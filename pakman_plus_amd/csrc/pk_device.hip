@@ -663,16 +663,21 @@ struct TileLds {
 // per input so that NT typical records fit.
 constexpr int L1_SB_MIN = 8192, L1_SB_MAX = 98304;  // staged bytes per chunk
 
-// Level-1 scatter blocks: 1024 threads for two-word keys on the main path
-// (two blocks per CU, <= 60 KB of LDS and <= 64 VGPRs each; k = 64
-// bacterial 1.50 -> 1.34 ms), 512 otherwise (one-word keys measured
-// 0.02-0.04 ms faster at 512; the rank-share and builder walks need more
-// registers).
+// Level-1 scatter blocks: 1024 threads on the main path (two blocks per CU,
+// <= 60 KB of LDS and <= 64 VGPRs each; fewer blocks per CU keep fewer
+// partly written lines open), 512 for the rank-share and builder walks,
+// which need more registers.  With one-word keys a scatter block takes the
+// reads of two histogram blocks (k = 32 bacterial 1.27 -> 1.22 ms; the
+// histogram itself is faster at 512 threads over half as many reads); with
+// two-word keys one (k = 64 bacterial 1.50 -> 1.34 ms against 512 threads).
 template <class Key, int ENC, bool OWN>
 struct L1Scatter {
-  static constexpr bool wide = sizeof(Key) == 16 && ENC == ENC_MAIN && !OWN;
-  static constexpr int NT = wide ? 1024 : 512;
-  static constexpr int SB_MAX = wide ? 131072 : L1_SB_MAX;
+  static constexpr bool main = ENC == ENC_MAIN && !OWN;
+  static constexpr int NT = main ? 1024 : 512;
+  static constexpr int SB_MAX = main ? 131072 : L1_SB_MAX;
+  // histogram blocks per scatter block (their output runs per digit are
+  // adjacent in the digit-major scan)
+  static constexpr int GF = main && sizeof(Key) == 8 ? 2 : 1;
 };
 
 struct ChunkLds {
@@ -908,8 +913,10 @@ template <class Key, int ENC, bool OWN>
 __global__ __launch_bounds__((L1Scatter<Key, ENC, OWN>::NT)) void k_l1_scatter(const uint8_t* fa, uint64_t n,
                                                       const ull* rstart, const uint32_t* rlen,
                                                       uint64_t R, int K, int KB, Key mask, int d1,
-                                                      int rpb, int sb, OwnCfg cf, const ull* O, Key* S1) {
+                                                      int rpb, int sb, OwnCfg cf, const ull* O,
+                                                      uint64_t nblk_h, Key* S1) {
   constexpr int NT = L1Scatter<Key, ENC, OWN>::NT;
+  constexpr int GF = L1Scatter<Key, ENC, OWN>::GF;
   __shared__ ChunkLds C;
   // dynamic LDS: plane A, 2^d1 output cursors, plane B, 2^d1 ranks
   extern __shared__ ull l1_dyn[];
@@ -918,11 +925,11 @@ __global__ __launch_bounds__((L1Scatter<Key, ENC, OWN>::NT)) void k_l1_scatter(c
   const Stage sg{l1_dyn, reinterpret_cast<uint32_t*>(base + nb), sb};
   uint32_t* h = sg.B + sb / 32;
   for (int i = threadIdx.x; i < nb; i += NT) {
-    base[i] = O[(ull)i * gridDim.x + blockIdx.x];
+    base[i] = O[(ull)i * nblk_h + (ull)blockIdx.x * GF];
     h[i] = 0;
   }
-  const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
-  const uint64_t r1 = r0 + rpb < R ? r0 + rpb : R;
+  const uint64_t r0 = (uint64_t)blockIdx.x * GF * rpb;
+  const uint64_t r1 = r0 + (uint64_t)GF * rpb < R ? r0 + (uint64_t)GF * rpb : R;
   const int bpos = KB - d1;
   // stage_chunk's barriers order the set-up above before the first walk
   for (uint64_t c0 = r0; c0 < r1;) {
@@ -2819,9 +2826,10 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
                      boff);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[4], st));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key, ENC, OWN>), dim3((unsigned)nblk), dim3(snt),
-                     l1_lds_bytes(sbs, (int)B1, 12), st, fa, n, rstart, rlen, R, K, KB, mask, d1, rpb, sbs, cf,
-                     O, S);
+  constexpr int sgf = L1Scatter<Key, ENC, OWN>::GF;
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key, ENC, OWN>), dim3((unsigned)((nblk + sgf - 1) / sgf)),
+                     dim3(snt), l1_lds_bytes(sbs, (int)B1, 12), st, fa, n, rstart, rlen, R, K, KB, mask, d1, rpb,
+                     sbs, cf, O, nblk, S);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[16], st));
   uint64_t B = B1;
