@@ -5,15 +5,19 @@ one MI355X.
 One step = pk_count + pk_cutoff + pk_macro_nodes over the whole read set of
 the workload, with the FASTA bytes already resident in HBM.  Prints one JSON
 line (metric, value in k-mer instances per second, roofline of the dominant
-kernel, CPU baseline of the oracle on a bounded sample).
+kernel, CPU baseline of the oracle on a bounded sample of the same reads).
 
-  python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload bacterial]
-                  [--forms plain,batched,served,formats] [--sets 256] [--slots 4] [--load 0.8]
+  python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload large]
+                  [--extra-workloads bacterial,small]
+                  [--forms plain,batched,served,formats] [--sets 256] [--slots 4]
 
-The batched and served forms run after the plain timed steps, on the
-256-set workload (see bench_batched_served), and are reported in the same
-JSON line under "batched" and "served"; "formats" times the same reads as
-FASTQ with '\r\n' line ends (bench_formats).
+The line's own figures are the large workload's (100 Mbp genome with repeat
+families, 150 bp reads, 50x, k = 64); the bacterial and small workloads are
+timed the same way and reported under "workloads".  The batched and served
+forms run after the plain timed steps, on the 256-set workload (see
+bench_batched_served), and are reported under "batched" and "served";
+"formats" times the bacterial reads as FASTQ with '\r\n' line ends
+(bench_formats).
 """
 from __future__ import annotations
 
@@ -27,6 +31,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # B/s, MI355X
+# share of the reads the CPU baseline counts, per workload (10-30 s of CPU work)
+CPU_FRAC = {"large": 0.03, "bacterial": 0.25, "small": 1.0}
 METRIC = "k-mers/s + % HBM peak, batched/served vs plain; p99 job latency at load"
 
 WORKLOADS = {
@@ -39,16 +45,23 @@ WORKLOADS = {
 }
 
 
-# stages timed around exactly one kernel launch -> that kernel's name
-KERNELS = {"hist": "k_l1_hist<{key}>", "l1_scatter": "k_l1_scatter<{key}>",
-           "sort": "k_bucket_hash<{key}>", "compact": "k_compact<{key}>"}
+# stages timed by HIP events around their kernel launches -> (kernel name,
+# timing field, launches per step: None = one; a name = that timing field)
+KERNELS = {"hist": ("k_l1_hist<{key}, 0, false>", "hist_ms", None),
+           "l1_scatter": ("k_l1_scatter<{key}, 0, false>", "l1_scatter_ms", None),
+           "ms_hist": ("k_ms_hist<{key}>", "ms_hist_ms", "ms_levels"),
+           "ms_scatter": ("{ms}<{key}>", "ms_scatter_ms", "ms_levels"),
+           "sort": ("k_bucket_hash<{key}>", "sort_ms", None),
+           "compact": ("k_compact<{key}>", "compact_ms", None)}
 
 
 def alg_bytes(N, U, V, M, S, W):
-    """Algorithmic HBM bytes per kernel (each datum read or written once)."""
+    """Algorithmic HBM bytes per kernel launch (each datum read or written once)."""
     return {
         "hist": S,                              # read the FASTA
         "l1_scatter": S + N * W,                # read the FASTA, write the instances
+        "ms_hist": N * W,                       # read the instances
+        "ms_scatter": 2 * N * W,                # read and write the instances
         "sort": N * W + U * (W + 4),            # read the instances, write (key, count)
         "compact": 2 * U * (W + 4),             # read and write the unique table
         "cutoff": U * 4 + U * (W + 4) + V * (W + 4),
@@ -57,6 +70,30 @@ def alg_bytes(N, U, V, M, S, W):
         "pipeline": (S + 2 * N * W + U * (W + 4) + U * 4 + U * (W + 4) + V * (W + 4)
                      + 2 * V * (W + 4) + M * (W + 32)),
     }
+
+
+def dominant_kernel(tm, N, U, V, M, S, W, workload):
+    """The kernel stage with the largest time per step, its time per launch and
+    achieved algorithmic bandwidth (alg_bytes per launch / time per launch)."""
+    ab = alg_bytes(N, U, V, M, S, W)
+    best = None
+    for stage, (_, field, per) in KERNELS.items():
+        total = float(tm.get(field, 0.0))
+        launches = int(tm.get(per, 0)) if per else 1
+        if total <= 0 or launches <= 0:
+            continue
+        if best is None or total > best[1]:
+            best = (stage, total, launches)
+    stage, total, launches = best
+    per_ms = total / launches
+    key = "unsigned long" if W == 8 else "unsigned __int128"
+    ms = "k_ms_scatter_p" if int(tm.get("ms_scatter_form", 0)) == 1 else "k_ms_scatter"
+    achieved = ab[stage] / (per_ms * 1e-3)
+    return {"bound": "hbm", "kernel": KERNELS[stage][0].format(key=key, ms=ms),
+            "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": load_traffic(stage, workload),
+            "alg_bytes_per_launch": ab[stage], "avg_ms": per_ms, "launches_per_step": launches,
+            "share_of_step": None}
 
 
 def cpu_baseline(fa, k, t, frac):
@@ -303,14 +340,83 @@ def copy_calibration(nbytes=1 << 30, reps=5):
     print(json.dumps({"copy_calibration_bytes_read": nbytes, "bytes_written": nbytes, "reps": reps}))
 
 
+def run_workload(name, steps, warmup, t):
+    """Plain steps of one workload, FASTA resident in HBM: its figures for the
+    JSON line (and the FASTA bytes, for the CPU baseline and the formats form)."""
+    import torch
+    from pakman_plus_amd import api, gen, tune
+    fn, k, desc = WORKLOADS[name]
+    spec = getattr(gen, fn)()
+    fa = gen.make_fasta(spec)
+    nbytes = int(fa.nbytes)
+    dev = torch.from_numpy(fa).to("cuda:0")
+    torch.cuda.synchronize()
+    ctx = api.Context(0)
+    tun = tune.for_size(tune.load(), nbytes, k)
+    if tun:
+        ctx.set_tuning(**tun)
+
+    def step():
+        ctx.count(None, k, device_ptr=dev.data_ptr(), nbytes=nbytes)
+        ctx.cutoff(t)
+        ctx.macro_nodes()
+        return ctx.timing()
+
+    for _ in range(warmup):
+        step()
+    keys = ("parse_ms", "hist_ms", "scatter_ms", "l1_scatter_ms", "ms_hist_ms", "ms_scatter_ms", "sort_ms",
+            "split_ms", "compact_ms", "cutoff_ms", "mn_ms")
+    acc = {x: 0.0 for x in keys}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tm = step()
+        for x in keys:
+            acc[x] += tm[x]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    N = int(tm["n_instances"])
+    U, V, M = ctx.table.n, ctx.n_kept, ctx.n_nodes
+    W = 8 if k <= 32 else 16
+    ms_per_step = elapsed * 1e3 / steps
+    avg = {x: acc[x] / steps for x in keys}
+    avg.update({"ms_levels": int(tm["ms_levels"]), "ms_scatter_form": int(tm["ms_scatter_form"])})
+    ab = alg_bytes(N, U, V, M, nbytes, W)
+    roof = dominant_kernel(avg, N, U, V, M, nbytes, W, name)
+    roof["share_of_step"] = roof["avg_ms"] * roof["launches_per_step"] / ms_per_step
+    ctx.close()
+    del dev
+    torch.cuda.empty_cache()
+    out = {
+        "value": N * steps / elapsed,
+        "ms_per_step": ms_per_step,
+        "steps": steps, "warmup": warmup,
+        "dtype": "u64" if k <= 32 else "u128 (two u64 words)",
+        "config": {"workload": desc, "k": k, "t": t, "reads": int(tm["n_reads"]),
+                   "kmer_instances": N, "unique_kmers": U, "kept_kmers": V, "macro_nodes": M,
+                   "fasta_bytes": nbytes, "bucket_bits": int(tm["bucket_bits"]),
+                   "partition_levels": 1 + int(tm["ms_levels"]),
+                   "split_buckets": int(tm["n_split"]), "parallelism": "1 GPU",
+                   "tuning": tun or "built-in"},
+        "roofline": roof,
+        "pipeline": {"alg_bytes": ab["pipeline"], "alg_bytes_per_instance": ab["pipeline"] / max(N, 1),
+                     "frac_of_peak": ab["pipeline"] / (ms_per_step * 1e-3) / HBM_PEAK,
+                     "stage_ms": {x.replace("_ms", ""): v for x, v in avg.items()}},
+    }
+    return out, fa, k, tun
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="bacterial", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="large", choices=sorted(WORKLOADS))
+    ap.add_argument("--extra-workloads", default="bacterial,small",
+                    help="workloads timed after the main one, reported under 'workloads' ('' for none)")
     ap.add_argument("--t", type=int, default=21)
-    ap.add_argument("--cpu-frac", type=float, default=0.25)
+    ap.add_argument("--cpu-frac", type=float, default=None,
+                    help="share of the reads the CPU baseline counts (default by workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--forms", default="plain,batched,served,formats",
@@ -326,96 +432,47 @@ def main():
               "exchange of this path is a design, not a build", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-    import torch
-    from pakman_plus_amd import api, gen
-
     if args.copy_calibration:
         copy_calibration()
         return
 
-    fn, k, desc = WORKLOADS[args.workload]
-    spec = getattr(gen, fn)()
-    fa = gen.make_fasta(spec)
-    nbytes = int(fa.nbytes)
-    dev = torch.from_numpy(fa).to("cuda:0")
-    torch.cuda.synchronize()
-    ctx = api.Context(0)
-    from pakman_plus_amd import tune
-    tuning_table = tune.load()
-    tun = tune.for_size(tuning_table, nbytes, k)
-    if tun:
-        ctx.set_tuning(**tun)
-
-    def step():
-        ctx.count(None, k, device_ptr=dev.data_ptr(), nbytes=nbytes)
-        ctx.cutoff(args.t)
-        ctx.macro_nodes()
-        return ctx.timing()
-
-    for _ in range(args.warmup):
-        step()
-    keys = ("parse_ms", "hist_ms", "scatter_ms", "l1_scatter_ms", "sort_ms", "split_ms", "compact_ms",
-            "cutoff_ms", "mn_ms")
-    acc = {x: 0.0 for x in keys}
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        tm = step()
-        for x in keys:
-            acc[x] += tm[x]
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    N = int(tm["n_instances"])
-    U = ctx.table.n
-    V = ctx.n_kept
-    M = ctx.n_nodes
-    W = 8 if k <= 32 else 16
-    ms_per_step = elapsed * 1e3 / args.steps
-    value = N * args.steps / elapsed
-    avg = {x.replace("_ms", ""): acc[x] / args.steps for x in keys}
-    ab = alg_bytes(N, U, V, M, nbytes, W)
-    # dominant kernel: the longest per-launch time among the stages timed
-    # around exactly one kernel launch
-    cand = {x: avg[x] for x in KERNELS if avg[x] > 0}
-    dom = max(cand, key=cand.get)
-    achieved = ab[dom] / (avg[dom] * 1e-3)
-    traffic = load_traffic(dom, args.workload)
+    main_out, fa, k, tun = run_workload(args.workload, args.steps, args.warmup, args.t)
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": main_out["value"],
         "unit": "k-mers/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": main_out["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u64" if k <= 32 else "u128 (two u64 words)",
+        "dtype": main_out["dtype"],
         "data": "synthetic (seeded generator, pakman_plus_amd.gen)",
-        "config": {"workload": desc, "k": k, "t": args.t, "reads": int(tm["n_reads"]),
-                   "kmer_instances": N, "unique_kmers": U, "kept_kmers": V, "macro_nodes": M,
-                   "fasta_bytes": nbytes, "bucket_bits": int(tm["bucket_bits"]),
-                   "split_buckets": int(tm["n_split"]), "parallelism": "1 GPU",
-                   "tuning": tun or "built-in"},
-        "roofline": {"bound": "hbm", "kernel": KERNELS[dom].format(key="unsigned long" if k <= 32 else "u128"),
-                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "alg_bytes_per_launch": ab[dom], "avg_ms": avg[dom]},
-        "pipeline": {"alg_bytes": ab["pipeline"], "frac_of_peak": ab["pipeline"] / (ms_per_step * 1e-3) / HBM_PEAK,
-                     "stage_ms": avg},
+        "config": main_out["config"],
+        "roofline": main_out["roofline"],
+        "pipeline": main_out["pipeline"],
     }
-    del dev
-    ctx.close()
     forms = set(args.forms.split(","))
-    if "formats" in forms and args.workload == "bacterial":
-        line["formats"] = bench_formats(fa, k, args.t, args.steps, args.warmup, tun, ms_per_step)
+    extras = [w for w in args.extra_workloads.split(",") if w and w != args.workload]
+    main_fa = fa
+    if extras:
+        line["workloads"] = {}
+        for w in extras:
+            sub, wfa, wk, wtun = run_workload(w, min(args.steps, 10), min(args.warmup, 2), args.t)
+            line["workloads"][w] = sub
+            if w == "bacterial" and "formats" in forms:
+                line["formats"] = bench_formats(wfa, wk, args.t, min(args.steps, 10), min(args.warmup, 2), wtun,
+                                                sub["ms_per_step"])
+            del wfa
+    elif args.workload == "bacterial" and "formats" in forms:
+        line["formats"] = bench_formats(fa, k, args.t, args.steps, args.warmup, tun, main_out["ms_per_step"])
     if forms & {"batched", "served"}:
         line.update(bench_batched_served(args))
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(fa, k, args.t, args.cpu_frac)
+        frac = args.cpu_frac if args.cpu_frac is not None else CPU_FRAC.get(args.workload, 0.25)
+        line["cpu_baseline"] = cpu_baseline(main_fa, k, args.t, frac)
     print(json.dumps(line), flush=True)
 
 

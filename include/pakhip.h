@@ -98,6 +98,11 @@ typedef struct pk_timing {
   int32_t n_split;      /* buckets that needed splitting */
   float l1_scatter_ms;  /* first scatter level alone (one kernel, part of scatter_ms) */
   float normalize_ms;   /* input-format normalization (pk_count_format; 0 otherwise) */
+  float ms_hist_ms;     /* further partition levels: their histogram kernels, summed */
+  float ms_scatter_ms;  /* further partition levels: their scatter kernels, summed */
+  int32_t ms_levels;    /* further partition levels of the last count (timed: first 4) */
+  int32_t ms_scatter_form; /* their scatter kernel: 1 persistent blocks, 2 one block
+                              per tile (the last level's; 0 when none ran) */
 } pk_timing;
 
 /* Create a context on HIP device `device`.  hbm_budget_bytes = 0 means no

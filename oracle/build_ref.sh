@@ -30,6 +30,8 @@ for K in "${KS[@]}"; do
   OBJ="$OUT/obj_k$K"
   mkdir -p "$OBJ"
   CXX="g++ -std=c++14 -O2 -fopenmp -w $DEFS -I$REF/includes -I$REF/src -I$MPI_HOME/include"
+  # the harness also takes the library's node-record filler (include/pakhip_mnmap.hpp)
+  HCXX="$CXX -I$HERE/../include"
   pids=()
   for tu in "${TUS[@]}"; do
     # distribute_kmers.cpp's main() is renamed so the harness can drive the
@@ -37,7 +39,7 @@ for K in "${KS[@]}"; do
     $CXX -Dmain=pakref_unused_main -c "$REF/src/$tu.cpp" -o "$OBJ/$tu.o" &
     pids+=($!)
   done
-  $CXX -c "$HERE/ref_harness.cpp" -o "$OBJ/ref_harness.o" &
+  $HCXX -c "$HERE/ref_harness.cpp" -o "$OBJ/ref_harness.o" &
   pids+=($!)
   for p in "${pids[@]}"; do wait "$p"; done
   g++ -fopenmp -static-libstdc++ -static-libgcc "$OBJ"/*.o "$MPI_HOME/lib/libmpi.so.12" -Wl,-rpath,"$MPI_HOME/lib" \

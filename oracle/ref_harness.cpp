@@ -16,7 +16,16 @@
 // Everything after counting is the reference's own code, called in the order
 // of src/distribute_kmers.cpp:132-215.  Extra dumps (macro nodes before and
 // after wiring) go to files named by $PAKREF_DUMP (default "pakref").
+//
+// With $PAKREF_NODES naming a file of node records (written by the tests:
+// "PKMN", u64 count, i32 k, then count pk_mnode records and count x 4
+// prefix-order bytes), counting and begin_mnode_construction are skipped:
+// the MN_map is filled from those records by the library's filler
+// (include/pakhip_mnmap.hpp, pakhip::fill_MN_map) and the reference's
+// wiring, compaction and walk run on it.  The contigs then show whether the
+// records plus the filler are a drop-in for construct_macro_nodes.
 
+#include <array>
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -24,10 +33,12 @@
 #include <vector>
 
 #include "distribute_kmers.h"
+#include "pakhip_mnmap.hpp"
 
 extern std::vector<KmerPairs> kmer_proc_buf;
 extern std::string inputFileName;
 extern int rank, size;
+extern int coverage;
 void parseCommandLine(const int argc, char* const argv[]);
 void set_num_threads();
 
@@ -105,6 +116,25 @@ static void dump_nodes(const std::vector<std::pair<kmer_t, MacroNode>>& mn, cons
   fclose(f);
 }
 
+// MN_map from a node-record file (see the header comment); false on a bad file.
+static bool fill_from_records(const char* path, std::vector<std::pair<kmer_t, MacroNode>>& MN_map) {
+  FILE* f = fopen(path, "rb");
+  if (!f) { perror(path); return false; }
+  char magic[4];
+  unsigned long long n = 0;
+  int k = 0;
+  bool ok = fread(magic, 1, 4, f) == 4 && memcmp(magic, "PKMN", 4) == 0 && fread(&n, 8, 1, f) == 1 &&
+            fread(&k, 4, 1, f) == 1 && k == KMER_LENGTH;
+  std::vector<pk_mnode> recs(ok ? n : 0);
+  std::vector<std::array<uint8_t, 4>> order(ok ? n : 0);
+  if (ok && n) ok = fread(recs.data(), sizeof(pk_mnode), n, f) == n && fread(order.data(), 4, n, f) == n;
+  fclose(f);
+  if (!ok) { fprintf(stderr, "%s: not a node-record file for k=%d\n", path, KMER_LENGTH); return false; }
+  pakhip::fill_MN_map<kmer_t, MacroNode>(recs.data(), recs.size(), k, coverage, order.data(), MN_map);
+  if (rank == 0) printf("Total number of macro_nodes across all proc's: %lu\n", (unsigned long)MN_map.size());
+  return true;
+}
+
 int main(int argc, char** argv) {
   MPI_Init(&argc, &argv);
   MPI_Comm_rank(MPI_COMM_WORLD, &rank);
@@ -114,6 +144,11 @@ int main(int argc, char** argv) {
   const char* dump = getenv("PAKREF_DUMP");
   std::string prefix = dump ? dump : "pakref";
 
+  std::vector<std::pair<kmer_t, MacroNode>> MN_map;
+  const char* recs = getenv("PAKREF_NODES");
+  if (recs) {
+    if (!fill_from_records(recs, MN_map)) return 2;
+  } else {
   FILE* fp = fopen(inputFileName.c_str(), "rb");
   if (!fp) { perror(inputFileName.c_str()); return 2; }
   fseek(fp, 0, SEEK_END);
@@ -125,8 +160,8 @@ int main(int argc, char** argv) {
 
   perform_kmer_counting(buf.data(), (size_t)sz);
 
-  std::vector<std::pair<kmer_t, MacroNode>> MN_map;
   begin_mnode_construction(MN_map);
+  }
   dump_nodes(MN_map, prefix + "_mn.txt", false);
   initiate_mnode_wiring(MN_map);
   dump_nodes(MN_map, prefix + "_wired.txt", true);

@@ -47,7 +47,9 @@ class _Timing(ctypes.Structure):
                 ("cutoff_ms", ctypes.c_float), ("mn_ms", ctypes.c_float),
                 ("n_instances", ctypes.c_uint64), ("n_reads", ctypes.c_uint64),
                 ("bucket_bits", ctypes.c_int32), ("n_split", ctypes.c_int32),
-                ("l1_scatter_ms", ctypes.c_float), ("normalize_ms", ctypes.c_float)]
+                ("l1_scatter_ms", ctypes.c_float), ("normalize_ms", ctypes.c_float),
+                ("ms_hist_ms", ctypes.c_float), ("ms_scatter_ms", ctypes.c_float),
+                ("ms_levels", ctypes.c_int32), ("ms_scatter_form", ctypes.c_int32)]
 
 
 class Tuning(ctypes.Structure):
@@ -248,6 +250,12 @@ class Context:
     @property
     def device_bytes(self) -> int:
         return int(self._lib.pk_ctx_device_bytes(self._h))
+
+    def release_buffers(self):
+        """Free the context's device buffers (they grow back on the next call)."""
+        self._check(self._lib.pk_ctx_release_buffers(self._h))
+        self.table = None
+        self.n_kept = self.n_nodes = 0
 
     def set_tuning(self, l1_chunks: int = 0, l1_bits: int = 0, bucket_factor: float = 0.0,
                    ms_scatter: int = 0):

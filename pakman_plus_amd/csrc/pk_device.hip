@@ -632,6 +632,7 @@ constexpr int MS_NT = 512;
 constexpr int L1_MAX_BITS = PK_TUNING_L1_BITS_MAX;  // level-1 digit width (direct scatter)
 constexpr int L1_DEFAULT_BITS = 8;  // wider digits keep more partial lines open
 constexpr int MS_MAX_BITS = 8;   // further levels (tile split into runs)
+constexpr int MS_TIMED_LEVELS = 4;  // further levels with their own timing events
 
 template <class Key>
 struct TileCfg {
@@ -2294,7 +2295,10 @@ struct pk_ctx {
   uint64_t held = 0;
   std::string err;
   DevBuf buf[B_COUNT];
-  hipEvent_t ev[20] = {};
+  // 0-19: stage boundaries; 20 + 4 L ..: further partition level L (hist
+  // start/end, scatter start/end) for the first MS_TIMED_LEVELS levels
+  hipEvent_t ev[20 + 4 * 4] = {};
+  int ms_levels = 0;  // further partition levels of the last count
   pk_timing timing{};
   // state of the current table
   int k = 0;
@@ -2744,6 +2748,7 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   c->words = sizeof(Key) / 8;
   c->timing.n_instances = N;
   c->timing.n_reads = R;
+  c->ms_levels = 0;
   PK_HIP(c, hipEventRecord(c->ev[1], st));
   Key *S, *TK;
   int32_t *C, *TC;
@@ -2855,8 +2860,10 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
     // may still be reading theirs
     PK_TRY(need(c, o ? B_HIST2 : B_HIST1, tiles << db, &H2));
     PK_TRY(need(c, o ? B_OFF2 : B_OFF1, (tiles << db) + 1, &O2));
+    const int lv = c->ms_levels < MS_TIMED_LEVELS ? c->ms_levels : -1;
     hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(1024), 0, st, boff, (int)B, TQ, tp);
     PK_HIP(c, hipGetLastError());
+    if (lv >= 0) PK_HIP(c, hipEventRecord(c->ev[20 + 4 * lv], st));
     // grid and scan over the upper bound of the level's tiles: blocks past
     // tp[B] return at once, and the scan of their (unused) histogram slots
     // only extends past every offset that is read
@@ -2865,6 +2872,7 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_hist<Key>), dim3((unsigned)ntiles), dim3(MS_NT), 0, st, Sin,
                        boff, tp, (int)B, dpos, db, H2);
     PK_HIP(c, hipGetLastError());
+    if (lv >= 0) PK_HIP(c, hipEventRecord(c->ev[21 + 4 * lv], st));
     PK_TRY(scan_excl<uint32_t>(c, H2, O2, ntiles << db));
     hipLaunchKernelGGL(k_ms_boff, dim3(grid_for(B2 + 1, 256)), dim3(256), 0, st, O2, tp, (int)B, db,
                        (ull)N, boff2);
@@ -2875,6 +2883,8 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
     // (k = 32, 2.5e8 instances) than one block per tile
     const ull pgrid = 2ull * (ull)c->ncu;
     const bool persistent = tun.ms_scatter ? tun.ms_scatter == 1 : ntiles >= 256 * pgrid;
+    if (lv >= 0) PK_HIP(c, hipEventRecord(c->ev[22 + 4 * lv], st));
+    c->timing.ms_scatter_form = persistent ? 1 : 2;
     if (persistent)
       hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_scatter_p<Key>), dim3((unsigned)pgrid), dim3(MS_NT), 0, st,
                          Sin, boff, tp, (int)B, dpos, db, O2, Sout);
@@ -2882,6 +2892,8 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
       hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_scatter<Key>), dim3((unsigned)ntiles), dim3(MS_NT), 0, st,
                          Sin, boff, tp, (int)B, dpos, db, O2, Sout);
     PK_HIP(c, hipGetLastError());
+    if (lv >= 0) PK_HIP(c, hipEventRecord(c->ev[23 + 4 * lv], st));
+    c->ms_levels++;
     std::swap(Sin, Sout);
     boff = boff2;
     B = B2;
@@ -3264,6 +3276,9 @@ int pk_ctx_release_buffers(pk_ctx* c) {
   }
   c->held = 0;
   c->have_table = c->have_kept = c->have_nodes = false;
+  c->have_norm = false;
+  c->norm_n = 0;
+  c->pend_count = c->pend_split = c->pend_cutoff = c->pend_mn = c->pend_norm = false;
   return PK_OK;
 }
 
@@ -3273,6 +3288,9 @@ static int fasta_to_device(pk_ctx* c, const char* fasta, uint64_t nbytes, int fa
 int pk_count(pk_ctx* c, const char* fasta, uint64_t nbytes, int k, int fasta_on_device,
              pk_table* out) {
   if (!c || !out || (!fasta && nbytes)) return PK_EINVAL;
+  // a failure before the count itself leaves no stale figures behind
+  c->timing = pk_timing{};
+  c->pend_count = c->pend_split = false;
   if (k < 8 || k > 64) return c->fail(PK_EINVAL, "k must be in [8, 64]");
   PK_HIP(c, hipSetDevice(c->device));
   const uint8_t* fa;
@@ -3413,6 +3431,9 @@ int pk_download_normalized(pk_ctx* c, char* host_out) {
 int pk_count_format(pk_ctx* c, const char* data, uint64_t nbytes, int k, int format, int data_on_device,
                     pk_table* out) {
   if (!c || !out || (!data && nbytes)) return PK_EINVAL;
+  // a failure before the count itself leaves no stale figures behind
+  c->timing = pk_timing{};
+  c->pend_count = c->pend_split = false;
   if (k < 8 || k > 64) return c->fail(PK_EINVAL, "k must be in [8, 64]");
   PK_HIP(c, hipSetDevice(c->device));
   if (format == PK_FMT_FASTA_STRICT) return pk_count(c, data, nbytes, k, data_on_device, out);
@@ -3433,6 +3454,9 @@ int pk_count_format(pk_ctx* c, const char* data, uint64_t nbytes, int k, int for
 int pk_count_rank_share(pk_ctx* c, const char* fasta, uint64_t nbytes, int k, int nranks, int rank,
                         int fasta_on_device, pk_table* out) {
   if (!c || !out || (!fasta && nbytes)) return PK_EINVAL;
+  // a failure before the count itself leaves no stale figures behind
+  c->timing = pk_timing{};
+  c->pend_count = c->pend_split = false;
   if (k < 8 || k > 64) return c->fail(PK_EINVAL, "k must be in [8, 64]");
   if (nranks < 1 || rank < 0 || rank >= nranks) return c->fail(PK_EINVAL, "need 0 <= rank < nranks");
   if (nranks == 1) return pk_count(c, fasta, nbytes, k, fasta_on_device, out);
@@ -3469,6 +3493,8 @@ int pk_builder_count(pk_ctx* c, const char* fasta, uint64_t nbytes, int klen, ui
   if (klen < 1) return c->fail(PK_EINVAL, "klen must be >= 1");
   if (read_len > 0xFFFFFFF0ull) return c->fail(PK_EINVAL, "read_len must be < 2^32 - 16");
   if (delimiter == skip_symbol) return c->fail(PK_EINVAL, "delimiter and skip_symbol must differ");
+  c->timing = pk_timing{};
+  c->pend_count = c->pend_split = false;
   const BldSyms sy{(uint8_t)delimiter, (uint8_t)skip_symbol};
   PK_HIP(c, hipSetDevice(c->device));
   const uint8_t* fa;
@@ -3499,8 +3525,24 @@ int pk_cutoff_threshold(pk_ctx* c, uint64_t threshold, uint64_t* n_kept) {
   if (!c || !n_kept) return PK_EINVAL;
   if (!c->have_table) return c->fail(PK_ESTATE, "pk_cutoff_threshold before a count");
   PK_HIP(c, hipSetDevice(c->device));
-  // counts are int32: a threshold at or above INT32_MAX keeps nothing
-  const int mb = threshold >= 0x7FFFFFFFull ? 0x7FFFFFFF : (int)threshold + 1;
+  if (threshold >= 0x7FFFFFFFull) {
+    // counts are int32: no count exceeds such a threshold, the kept table is empty
+    PK_HIP(c, hipStreamSynchronize(c->stream));
+    int32_t* KC;
+    uint64_t* KK;
+    PK_TRY(need(c, B_KK, c->words, &KK));
+    PK_TRY(need(c, B_KC2, 1, &KC));
+    c->V = 0;
+    c->t = 1;
+    c->mb = 0;
+    c->have_kept = true;
+    c->have_nodes = false;
+    c->pend_cutoff = false;
+    c->timing.cutoff_ms = 0.f;
+    *n_kept = 0;
+    return PK_OK;
+  }
+  const int mb = (int)threshold + 1;
   int mbo = 0;
   return c->words == 1 ? cutoff_impl<uint64_t>(c, 1, nullptr, &mbo, n_kept, mb)
                        : cutoff_impl<u128>(c, 1, nullptr, &mbo, n_kept, mb);
@@ -3595,6 +3637,13 @@ int pk_last_timing(const pk_ctx* cc, pk_timing* out) {
     c->timing.sort_ms = el(6, 7);
     c->timing.split_ms = c->pend_split ? el(8, 9) : 0.f;
     c->timing.compact_ms = el(10, 11);
+    const int nl = c->ms_levels < MS_TIMED_LEVELS ? c->ms_levels : MS_TIMED_LEVELS;
+    c->timing.ms_hist_ms = c->timing.ms_scatter_ms = 0.f;
+    for (int l = 0; l < nl; l++) {
+      c->timing.ms_hist_ms += el(20 + 4 * l, 21 + 4 * l);
+      c->timing.ms_scatter_ms += el(22 + 4 * l, 23 + 4 * l);
+    }
+    c->timing.ms_levels = c->ms_levels;
     c->pend_count = false;
   }
   if (c->pend_cutoff) {

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <new>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -34,6 +35,7 @@ struct Job {
   pk_job spec{};
   uint64_t ticket = 0;
   int state = 0;  // 0 queued, 1 running, 2 done
+  int readers = 0;  // pk_server_download calls copying the results now
   pk_job_result res{};
   std::vector<uint64_t> hist;
   void* mem = nullptr;
@@ -83,12 +85,18 @@ struct pk_server {
     if (running == 0) return true;
     return running_est + resident + queue.front()->res.est_bytes <= budget;
   }
+  // err is written under mu (callers hold it, or use fail_unlocked)
   int fail(int code, const std::string& m) {
     err = m;
     return code;
   }
+  int fail_unlocked(int code, const std::string& m) {
+    std::lock_guard<std::mutex> lk(mu);
+    return fail(code, m);
+  }
   void worker(int slot);
   void run(int slot, Job* j);
+  void run_guarded(int slot, Job* j);
 };
 
 void pk_server::run(int slot, Job* j) {
@@ -111,14 +119,15 @@ void pk_server::run(int slot, Job* j) {
   pk_ctx_set_tuning(c, &tu);
   pk_table t{};
   int rc = pk_count(c, j->spec.fasta, j->spec.nbytes, j->spec.k, j->spec.fasta_on_device, &t);
-  pk_timing tm{};
-  pk_last_timing(c, &tm);
-  r.n_reads = tm.n_reads;
-  r.n_instances = tm.n_instances;
   if (rc) {
     set_err(rc);
     return;
   }
+  // the context's timing describes this job only once its count succeeded
+  pk_timing tm{};
+  pk_last_timing(c, &tm);
+  r.n_reads = tm.n_reads;
+  r.n_instances = tm.n_instances;
   pk_table kt{};
   pk_mn_table mt{};
   int crc = PK_OK;
@@ -171,6 +180,20 @@ void pk_server::run(int slot, Job* j) {
   r.nodes = pk_mn_table{mt.n, t.k, (const pk_mnode*)(m + o5)};
 }
 
+// run() with host allocation failures (the histogram vector) turned into the
+// job's PK_ENOMEM: nothing may throw out of a worker thread.
+void pk_server::run_guarded(int slot, Job* j) {
+  try {
+    run(slot, j);
+  } catch (const std::bad_alloc&) {
+    j->res.status = PK_ENOMEM;
+    std::snprintf(j->res.message, sizeof(j->res.message), "host allocation failed");
+  } catch (...) {
+    j->res.status = PK_EINVAL;
+    std::snprintf(j->res.message, sizeof(j->res.message), "unexpected host exception");
+  }
+}
+
 void pk_server::worker(int slot) {
   (void)hipSetDevice(device);
   const uint64_t slot_keep = budget / (uint64_t)(n_slots > 0 ? n_slots : 1);
@@ -187,7 +210,7 @@ void pk_server::worker(int slot) {
       running_est += j->res.est_bytes;
       j->res.t_start = now();
     }
-    run(slot, j);
+    run_guarded(slot, j);
     // a slot that grew past its share gives its buffers back
     if (pk_ctx_device_bytes(ctx[slot]) > slot_keep) (void)pk_ctx_release_buffers(ctx[slot]);
     {
@@ -275,8 +298,12 @@ double pk_server_now(const pk_server* s) { return s ? s->now() : 0.0; }
 
 const char* pk_server_last_error(const pk_server* s) { return s ? s->err.c_str() : "null server"; }
 
+// Largest cutoff bucket count (pk_cutoff's limit).
+constexpr int SERVER_T_MAX = 4096;
+
 static int submit_locked(pk_server* s, const pk_job& spec, uint64_t* ticket) {
   if (spec.k < 8 || spec.k > 64) return s->fail(PK_EINVAL, "k must be in [8, 64]");
+  if (spec.t > SERVER_T_MAX) return s->fail(PK_EINVAL, "t must be <= 4096");
   if (!spec.fasta && spec.nbytes) return s->fail(PK_EINVAL, "null fasta");
   if (spec.want_nodes && spec.t < 0) return s->fail(PK_EINVAL, "macro nodes need a cutoff (t >= 0)");
   Job* j = new Job();
@@ -293,9 +320,11 @@ static int submit_locked(pk_server* s, const pk_job& spec, uint64_t* ticket) {
 int pk_server_submit(pk_server* s, const pk_job* job, uint64_t* ticket) {
   if (!s || !job || !ticket) return PK_EINVAL;
   int rc;
-  {
+  try {
     std::lock_guard<std::mutex> lk(s->mu);
     rc = submit_locked(s, *job, ticket);
+  } catch (const std::bad_alloc&) {
+    return s->fail_unlocked(PK_ENOMEM, "host allocation failed");
   }
   s->cv_q.notify_all();
   return rc;
@@ -303,17 +332,19 @@ int pk_server_submit(pk_server* s, const pk_job* job, uint64_t* ticket) {
 
 int pk_server_submit_batch(pk_server* s, const pk_job* jobs, int n, uint64_t* tickets) {
   if (!s || n < 0 || (n && (!jobs || !tickets))) return PK_EINVAL;
-  std::vector<int> order((size_t)n);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return jobs[a].nbytes > jobs[b].nbytes; });
   int rc = PK_OK;
-  {
+  try {
+    std::vector<int> order((size_t)n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return jobs[a].nbytes > jobs[b].nbytes; });
     std::lock_guard<std::mutex> lk(s->mu);
     for (int i : order) {
       rc = submit_locked(s, jobs[i], &tickets[i]);
       if (rc) break;
     }
+  } catch (const std::bad_alloc&) {
+    rc = s->fail_unlocked(PK_ENOMEM, "host allocation failed");
   }
   s->cv_q.notify_all();
   return rc;
@@ -352,23 +383,30 @@ int pk_server_download(pk_server* s, uint64_t ticket, uint64_t* keys, int32_t* c
     if (it == s->jobs.end()) return s->fail(PK_EINVAL, "unknown ticket");
     j = it->second;
     if (j->state != 2) return s->fail(PK_ESTATE, "job has not finished");
+    // pk_server_release of this ticket waits until the copy below is done
+    j->readers++;
   }
   const pk_job_result& r = j->res;
-  if (hipSetDevice(s->device) != hipSuccess) return s->fail(PK_EHIP, "hipSetDevice failed");
   const uint64_t w = (uint64_t)r.table.words * 8;
   auto get = [&](void* dst, const void* src, uint64_t bytes) -> hipError_t {
     if (!dst || !bytes || !src) return hipSuccess;
     return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
   };
-  hipError_t e = hipSuccess;
+  hipError_t e = hipSetDevice(s->device);
   if (e == hipSuccess) e = get(keys, r.table.keys, r.table.n * w);
   if (e == hipSuccess) e = get(counts, r.table.counts, r.table.n * 4);
   if (e == hipSuccess) e = get(kept_keys, r.kept.keys, r.kept.n * w);
   if (e == hipSuccess) e = get(kept_counts, r.kept.counts, r.kept.n * 4);
   if (e == hipSuccess) e = get(nodes, r.nodes.nodes, r.nodes.n * sizeof(pk_mnode));
-  if (e != hipSuccess) return s->fail(PK_EHIP, std::string("download failed: ") + hipGetErrorString(e));
-  if (hist_t && !j->hist.empty()) std::memcpy(hist_t, j->hist.data(), j->hist.size() * 8);
-  return PK_OK;
+  if (e == hipSuccess && hist_t && !j->hist.empty()) std::memcpy(hist_t, j->hist.data(), j->hist.size() * 8);
+  int rc = PK_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    j->readers--;
+    if (e != hipSuccess) rc = s->fail(PK_EHIP, std::string("download failed: ") + hipGetErrorString(e));
+  }
+  s->cv_done.notify_all();
+  return rc;
 }
 
 int pk_server_set_tuning(pk_server* s, const pk_tune_entry* entries, int n) {
@@ -381,11 +419,15 @@ int pk_server_set_tuning(pk_server* s, const pk_tune_entry* entries, int n) {
         t.ms_scatter < 0 || t.ms_scatter > 2)
       return s->fail(PK_EINVAL, "tuning entry " + std::to_string(i) + " out of range");
   }
-  std::vector<pk_tune_entry> t(entries, entries + n);
-  std::sort(t.begin(), t.end(),
-            [](const pk_tune_entry& a, const pk_tune_entry& b) { return a.max_bytes < b.max_bytes; });
-  std::lock_guard<std::mutex> lk(s->mu);
-  s->tune = std::move(t);
+  try {
+    std::vector<pk_tune_entry> t(entries, entries + n);
+    std::sort(t.begin(), t.end(),
+              [](const pk_tune_entry& a, const pk_tune_entry& b) { return a.max_bytes < b.max_bytes; });
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->tune = std::move(t);
+  } catch (const std::bad_alloc&) {
+    return s->fail_unlocked(PK_ENOMEM, "host allocation failed");
+  }
   return PK_OK;
 }
 
@@ -393,11 +435,19 @@ int pk_server_release(pk_server* s, uint64_t ticket) {
   if (!s) return PK_EINVAL;
   Job* j;
   {
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->jobs.find(ticket);
     if (it == s->jobs.end()) return s->fail(PK_EINVAL, "unknown ticket");
+    if (it->second->state != 2) return s->fail(PK_ESTATE, "job has not finished");
+    // a download of the same ticket in progress finishes first (the job is
+    // looked up again: a concurrent release may have taken it meanwhile)
+    s->cv_done.wait(lk, [&] {
+      auto jt = s->jobs.find(ticket);
+      return jt == s->jobs.end() || jt->second->readers == 0;
+    });
+    it = s->jobs.find(ticket);
+    if (it == s->jobs.end()) return s->fail(PK_EINVAL, "unknown ticket");
     j = it->second;
-    if (j->state != 2) return s->fail(PK_ESTATE, "job has not finished");
     s->jobs.erase(it);
     s->resident -= j->mem_bytes;
   }
