@@ -148,6 +148,24 @@ int pk_ctx_release_buffers(pk_ctx* ctx);
 int pk_count(pk_ctx* ctx, const char* fasta, uint64_t nbytes, int k, int fasta_on_device,
              pk_table* out);
 
+/* Allocation-free count.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  pk_count's table of a
+ * device-resident FASTA buffer, written into the caller's device arrays
+ * (keys_dev: capacity * words uint64, counts_dev: capacity int32; words = 1
+ * for k <= 32, else 2); *n_out = distinct k-mers.  Every intermediate buffer
+ * sits in a static scratch array of PK_NOALLOC_SCRATCH_BYTES per GPU (held
+ * until the process ends), so no hipMalloc / hipFree is called.  Fails with
+ * PK_ENOMEM when the input's buffers exceed the scratch (about 4 MiB of
+ * 100 bp-read FASTA at k = 32) or the table exceeds capacity.  All calls on
+ * one GPU share its scratch array: calls that overlap on one GPU (from
+ * different contexts or threads) race; calls on one context are ordered.
+ * The table's last kernels may still run on the context's stream at return.
+ * The context's own table, kept table and nodes are invalidated. */
+#define PK_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int pk_count_noalloc(pk_ctx* ctx, const char* fasta_dev, uint64_t nbytes, int k, uint64_t* keys_dev,
+                     int32_t* counts_dev, uint64_t capacity, uint64_t* n_out);
+
 /* Input formats beyond the reference's strict single-line FASTA (no
  * reference counterpart: the reference reads only that format,
  * src/input_reading.cpp:69-312 and src/kmer_counting.cpp:2920-2945).  The

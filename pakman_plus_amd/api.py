@@ -87,6 +87,7 @@ MNODE_DTYPE = np.dtype([("key_lo", "<u8"), ("key_hi", "<u8"), ("pre_cnt", "<i4",
 # every symbol declared in include/pakhip.h
 EXPORTS = ("pk_ctx_create", "pk_ctx_destroy", "pk_last_error", "pk_ctx_stream",
            "pk_ctx_device_bytes", "pk_ctx_set_tuning", "pk_ctx_release_buffers", "pk_count",
+           "pk_count_noalloc",
            "pk_count_format", "pk_normalize", "pk_download_normalized",
            "pk_count_rank_share",
            "pk_lmer_frequency", "pk_builder_count",
@@ -130,6 +131,8 @@ def load_library():
     L.pk_ctx_device_bytes.restype = u64
     L.pk_count.argtypes = [vp, vp, u64, i32, i32, ctypes.POINTER(_Table)]
     L.pk_count.restype = i32
+    L.pk_count_noalloc.argtypes = [vp, vp, u64, i32, vp, vp, u64, ctypes.POINTER(u64)]
+    L.pk_count_noalloc.restype = i32
     L.pk_count_format.argtypes = [vp, vp, u64, i32, i32, i32, ctypes.POINTER(_Table)]
     L.pk_count_format.restype = i32
     L.pk_normalize.argtypes = [vp, vp, u64, i32, i32, ctypes.POINTER(u64)]
@@ -276,6 +279,20 @@ class Context:
         self.k = k
         self.table = TableView(int(t.n), int(t.words), int(t.k), t.keys or 0, t.counts or 0)
         return self.table
+
+    def count_noalloc(self, device_ptr: int, nbytes: int, k: int, keys_ptr: int, counts_ptr: int,
+                      capacity: int) -> int:
+        """pk_count's table of a device-resident FASTA written into the
+        caller's device arrays (keys: capacity * words uint64, counts:
+        capacity int32) with no device allocation; returns the number of
+        distinct k-mers.  Inputs must fit the library's static scratch
+        (PK_NOALLOC_SCRATCH_BYTES per GPU)."""
+        n = ctypes.c_uint64()
+        self._check(self._lib.pk_count_noalloc(self._h, ctypes.c_void_p(device_ptr), nbytes, k,
+                                               ctypes.c_void_p(keys_ptr), ctypes.c_void_p(counts_ptr),
+                                               capacity, ctypes.byref(n)))
+        self.table = None
+        return int(n.value)
 
     def count_format(self, data, k: int, fmt: str = "fasta", device_ptr: int | None = None,
                      nbytes: int | None = None) -> TableView:

@@ -42,6 +42,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -2317,6 +2318,13 @@ struct pk_ctx {
   uint64_t norm_n = 0;     // bytes of the last rewritten input (B_NORM)
   bool have_norm = false;
   pk_tuning tuning{};  // zeros: built-in choices
+  // Scratch mode of pk_count_noalloc: need() carves buffers from `arena`
+  // (the device's static scratch array) instead of allocating; the caller's
+  // output arrays stand in for the table (ids marked in arena_fixed).
+  bool arena_mode = false;
+  char* arena = nullptr;
+  uint64_t arena_cap = 0, arena_used = 0;
+  bool arena_fixed[B_COUNT] = {};
 
   int fail(int code, const std::string& what) {
     err = what;
@@ -2346,6 +2354,25 @@ template <class T>
 int need(pk_ctx* c, int id, uint64_t count, T** out) {
   DevBuf& b = c->buf[id];
   const uint64_t bytes = count * sizeof(T) + 256;
+  if (c->arena_mode) {
+    if (c->arena_fixed[id]) {
+      // a caller's output array: it must hold count entries (an empty
+      // table asks for one entry and writes none)
+      if (count * sizeof(T) > b.cap && !(count == 1 && c->U == 0))
+        return c->fail(PK_ENOMEM, "output arrays hold fewer entries than the table (" + std::to_string(count) +
+                                      " needed)");
+    } else if (b.cap < bytes) {
+      const uint64_t take = (bytes + 255) & ~255ull;
+      if (c->arena_used + take > c->arena_cap)
+        return c->fail(PK_ENOMEM, "input needs more than the " + std::to_string(c->arena_cap >> 20) +
+                                      " MiB scratch of pk_count_noalloc");
+      b.p = c->arena + c->arena_used;
+      b.cap = take;
+      c->arena_used += take;
+    }
+    *out = reinterpret_cast<T*>(b.p);
+    return PK_OK;
+  }
   if (b.cap < bytes) {
     if (b.p) {
       hipError_t e = hipFree(b.p);
@@ -2741,7 +2768,8 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
                   uint64_t R, uint64_t N, int K, int KB, Key mask, OwnCfg cf = OwnCfg{nullptr, 1, 0}) {
   constexpr int CAP = HashCfg<Key>::LIMIT;
   hipStream_t st = c->stream;
-  ull* small = reinterpret_cast<ull*>(c->buf[B_SMALL].p);
+  ull* small;
+  PK_TRY(need(c, B_SMALL, 16, &small));
   c->R = R;
   c->N = N;
   c->k = K;
@@ -3302,6 +3330,73 @@ int pk_count(pk_ctx* c, const char* fasta, uint64_t nbytes, int k, int fasta_on_
   out->k = c->k;
   out->keys = reinterpret_cast<const uint64_t*>(c->buf[B_TK].p);
   out->counts = reinterpret_cast<const int32_t*>(c->buf[B_TC].p);
+  return PK_OK;
+}
+
+// ---- Allocation-free count (pk_count_noalloc) --------------------------------
+// Every intermediate buffer of the count lives in one file-scope device array
+// per GPU; its address is taken once per device with hipGetSymbolAddress and
+// only that address is handed to kernels and copies.
+constexpr uint64_t NOALLOC_SCRATCH = PK_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH];
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[MAX_DEVICES] = {};
+
+static int scratch_address(pk_ctx* c, char** out) {
+  if (c->device < 0 || c->device >= MAX_DEVICES) return c->fail(PK_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[c->device]) {
+    void* p = nullptr;
+    // the calling thread has made c->device current
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return c->hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[c->device] = p;
+  }
+  *out = static_cast<char*>(g_scratch_ptr[c->device]);
+  return PK_OK;
+}
+
+int pk_count_noalloc(pk_ctx* c, const char* fasta_dev, uint64_t nbytes, int k, uint64_t* keys_dev,
+                     int32_t* counts_dev, uint64_t capacity, uint64_t* n_out) {
+  if (!c || !n_out || (!fasta_dev && nbytes) || (capacity && (!keys_dev || !counts_dev))) return PK_EINVAL;
+  if (k < 8 || k > 64) return c->fail(PK_EINVAL, "k must be in [8, 64]");
+  *n_out = 0;
+  PK_HIP(c, hipSetDevice(c->device));
+  char* arena = nullptr;
+  PK_TRY(scratch_address(c, &arena));
+  // the scratch array is shared by every call on this device: a call still
+  // running on this context's stream finishes before its contents change
+  PK_HIP(c, hipStreamSynchronize(c->stream));
+  c->timing = pk_timing{};
+  c->pend_count = c->pend_split = false;
+  const int words = k <= 32 ? 1 : 2;
+  // the context's own buffers step aside for the call
+  DevBuf saved[B_COUNT];
+  for (int i = 0; i < B_COUNT; i++) {
+    saved[i] = c->buf[i];
+    c->buf[i] = DevBuf{};
+    c->arena_fixed[i] = false;
+  }
+  c->buf[B_TK] = DevBuf{keys_dev, capacity * (uint64_t)words * 8};
+  c->buf[B_TC] = DevBuf{counts_dev, capacity * 4};
+  c->arena_fixed[B_TK] = c->arena_fixed[B_TC] = true;
+  c->arena = arena;
+  c->arena_cap = NOALLOC_SCRATCH;
+  c->arena_used = 0;
+  c->arena_mode = true;
+  c->U = 0;
+  const uint8_t* fa = reinterpret_cast<const uint8_t*>(fasta_dev);
+  int rc = (k <= 32) ? count_impl<uint64_t>(c, fa, nbytes, k) : count_impl<u128>(c, fa, nbytes, k);
+  c->arena_mode = false;
+  for (int i = 0; i < B_COUNT; i++) {
+    c->buf[i] = saved[i];
+    c->arena_fixed[i] = false;
+  }
+  // the table lives in the caller's arrays, not in the context
+  c->have_table = c->have_kept = c->have_nodes = false;
+  if (rc) return rc;
+  *n_out = c->U;
   return PK_OK;
 }
 
