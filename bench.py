@@ -5,7 +5,8 @@ one MI355X.
 One step = pk_count + pk_cutoff + pk_macro_nodes over the whole read set of
 the workload, with the FASTA bytes already resident in HBM.  Prints one JSON
 line (metric, value in k-mer instances per second, roofline of the dominant
-kernel, CPU baseline of the oracle on a bounded sample of the same reads).
+kernel, CPU baseline: the oracle's restatement of the reference at P ranks
+on the host's cores, and at one rank, on bounded samples of the same reads).
 
   python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload large]
                   [--extra-workloads bacterial,small]
@@ -31,8 +32,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # B/s, MI355X
-# share of the reads the CPU baseline counts, per workload (10-30 s of CPU work)
-CPU_FRAC = {"large": 0.03, "bacterial": 0.25, "small": 1.0}
+# shares of the reads the CPU baseline takes per workload, (all ranks, one
+# rank): each leg about 10-20 s of host time
+CPU_FRAC = {"large": (0.1, 0.01), "bacterial": (1.0, 0.1), "small": (1.0, 1.0)}
 METRIC = "k-mers/s + % HBM peak, batched/served vs plain; p99 job latency at load"
 
 WORKLOADS = {
@@ -96,29 +98,40 @@ def dominant_kernel(tm, N, U, V, M, S, W, workload):
             "share_of_step": None}
 
 
-def cpu_baseline(fa, k, t, frac):
-    """Oracle (C++ restatement of the reference's algorithm, 1 thread) on the
-    first `frac` of the reads: count + cutoff + macro nodes."""
+def _read_sample(arr, frac):
+    """The first `frac` of the reads of a strict single-line FASTA: (bytes, reads, all reads)."""
+    import numpy as np
+    nl = np.flatnonzero(arr == 10)
+    R = nl.size // 2
+    take = max(1, min(R, int(round(R * frac))))
+    return arr[:int(nl[2 * take - 1]) + 1], take, R
+
+
+def cpu_baseline(fa, k, t, frac_ranks, frac_one):
+    """The reference's algorithm on the host (the oracle's restatement of
+    count -> cutoff -> macro nodes at P ranks, one thread per rank, as
+    mpiexec -np P with OMP_NUM_THREADS=1): P = the host cores given to one GPU
+    (at most 16) on the first frac_ranks of the reads, and one rank on the
+    first frac_one of them."""
     import numpy as np
     from oracle import oracle as O
     O.build()
     arr = np.asarray(fa)
-    nl = np.flatnonzero(arr == 10)
-    R = nl.size // 2
-    take = max(1, int(R * frac))
-    end = int(nl[2 * take - 1]) + 1
-    sample = arr[:end]
-    t0 = time.perf_counter()
-    keys, counts = O.count(sample, k)
-    hist, mb, keep, rc = O.cutoff(counts, t)
-    if rc == 0:
-        O.macro_nodes(keys[keep], counts[keep], k)
-    dt = time.perf_counter() - t0
-    n_inst = int(counts.astype(np.int64).sum())
-    return {"value": n_inst / dt, "unit": "k-mers/s", "cores": 1, "kind": "port",
-            "sample": f"first {take} of {R} reads ({frac:.0%}) of the same workload: "
-                      f"{n_inst} k-mer instances, count + cutoff + macro nodes, 1 thread, "
-                      f"{dt:.1f} s"}
+    ranks = max(1, min(16, os.cpu_count() or 1))
+    out = {}
+    for key, P, frac in (("ranks", ranks, frac_ranks), ("np1", 1, frac_one)):
+        sample, take, R = _read_sample(arr, frac)
+        t0 = time.perf_counter()
+        r = O.pipeline_ranks(sample, k, t, P)
+        dt = time.perf_counter() - t0
+        out[key] = {"value": r["instances"] / dt, "unit": "k-mers/s", "cores": P, "seconds": dt,
+                    "sample": f"first {take} of {R} reads ({frac:.0%}) of the same workload: "
+                              f"{r['instances']} k-mer instances, {r['distinct']} distinct, {r['kept']} kept, "
+                              f"{r['nodes']} macro nodes; count + cutoff + macro nodes at {P} rank(s)"}
+    res = dict(out["ranks"])
+    res["kind"] = "port"
+    res["np1"] = out["np1"]
+    return res
 
 
 def make_batched_sets(num_sets, threads=8):
@@ -416,7 +429,8 @@ def main():
                     help="workloads timed after the main one, reported under 'workloads' ('' for none)")
     ap.add_argument("--t", type=int, default=21)
     ap.add_argument("--cpu-frac", type=float, default=None,
-                    help="share of the reads the CPU baseline counts (default by workload)")
+                    help="share of the reads the CPU baseline's all-rank leg counts (the one-rank "
+                         "leg a tenth of it; default by workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--forms", default="plain,batched,served,formats",
@@ -471,8 +485,10 @@ def main():
     if forms & {"batched", "served"}:
         line.update(bench_batched_served(args))
     if not args.no_cpu_baseline:
-        frac = args.cpu_frac if args.cpu_frac is not None else CPU_FRAC.get(args.workload, 0.25)
-        line["cpu_baseline"] = cpu_baseline(main_fa, k, args.t, frac)
+        fr, f1 = CPU_FRAC.get(args.workload, (0.25, 0.05))
+        if args.cpu_frac is not None:
+            fr, f1 = args.cpu_frac, args.cpu_frac / 10
+        line["cpu_baseline"] = cpu_baseline(main_fa, k, args.t, fr, f1)
     print(json.dumps(line), flush=True)
 
 

@@ -205,6 +205,15 @@ int pk_download_normalized(pk_ctx* ctx, char* host_out);
 int pk_count_rank_share(pk_ctx* ctx, const char* fasta, uint64_t nbytes, int k, int nranks, int rank,
                         int fasta_on_device, pk_table* out);
 
+/* One rank's share at nranks ranks when a k-mer belongs to rank
+ * retrieve_proc_id(kmer) (src/distribute_kmers.cpp:95-120): the owner rule
+ * of the -p counters (src/kmer_counting.cpp:1949-1950, 2136, 2320), hash31
+ * (includes/distribute_kmers.h:856-873) of the k-mer for k <= 32 and uhash31
+ * (:836-854, 128-bit) for k > 32, modulo nranks.  The shares partition
+ * pk_count's table.  nranks == 1 is pk_count. */
+int pk_count_rank_share_by_key(pk_ctx* ctx, const char* fasta, uint64_t nbytes, int k, int nranks, int rank,
+                               int fasta_on_device, pk_table* out);
+
 /* The 8-mer frequency table of the input (4^8 uint64 entries, host array),
  * as Sliding_window_l computes it (src/kmer_counting.cpp:391-438). */
 int pk_lmer_frequency(pk_ctx* ctx, const char* fasta, uint64_t nbytes, int fasta_on_device,
@@ -229,6 +238,29 @@ int pk_builder_count(pk_ctx* ctx, const char* fasta, uint64_t nbytes, int klen, 
                      int kmer_bits, char delimiter, char skip_symbol, int fasta_on_device,
                      pk_table* out, uint64_t* n_reads);
 
+/* Window counting of the builder API's implicit counters
+ * (Concurrent_Kmer_Partial_Counter::concurrent_count_and_process_omp_implicit_v0
+ * and _v1, src/concurrent_kmer_counter.hpp:910-1022 and 1363-1578): the same
+ * records as pk_builder_count (every record counted, whatever its length),
+ * every klen-byte window of each read, windows compared as the reference's
+ * strncmp(a, b, klen) does (raw bytes up to the first NUL; no 2-bit
+ * packing).  *n_reads = records (the reference's return value), *n_windows =
+ * windows (its permutation size), *n_distinct = distinct windows.
+ * PK_WINDOWS_V0 fails with PK_EFORMAT ("Variable size read") when a read is
+ * cut short by the end of the buffer (the reference throws, :965-967);
+ * PK_WINDOWS_V1 takes such reads but fails with PK_EFORMAT when one is
+ * shorter than klen (the reference's window count wraps there, :1427).
+ * read_len < klen fails with PK_EINVAL.  Windows are keyed by the rank of
+ * each byte value among those present in the reads, so klen x bits per value
+ * must fit 128 bits (e.g. klen = 32 with up to 16 distinct byte values);
+ * otherwise PK_EINVAL.  The context's table then holds those keys (no macro
+ * nodes on it). */
+#define PK_WINDOWS_V0 0
+#define PK_WINDOWS_V1 1
+int pk_builder_count_windows(pk_ctx* ctx, const char* fasta, uint64_t nbytes, int klen, uint64_t read_len,
+                             char delimiter, char skip_symbol, int mode, int fasta_on_device, uint64_t* n_reads,
+                             uint64_t* n_windows, uint64_t* n_distinct);
+
 /* Histogram cutoff on the current table (kmer_histogram.cpp:84-140):
  * hist_t[c] (host array of t entries) = number of k-mers with count c < t;
  * *min_bucket = first argmin of hist_t[1..t-1] (t == 1 -> 1); the kept
@@ -248,6 +280,15 @@ int pk_kept_table(pk_ctx* ctx, pk_table* out);
 
 /* Macro nodes of the kept table (mn_construct.cpp:212-477, one rank). */
 int pk_macro_nodes(pk_ctx* ctx, pk_mn_table* out);
+
+/* Macro nodes of the kept table as rank `rank` holds them at nranks ranks
+ * (construct_macro_nodes, src/mn_construct.cpp:212-477): the nodes whose
+ * (k-1)-mer key the rank owns, retrieve_proc_id(key) % nranks (hash31 for
+ * k <= 32, uhash31 for k > 32; mn_construct.cpp:234-247), each with the
+ * records pk_macro_nodes gives it (the regroup sends a node's owner every
+ * kept k-mer of the node, src/kmer_histogram.cpp:140-170).  The context's
+ * node table becomes the share. */
+int pk_macro_nodes_rank_share(pk_ctx* ctx, int nranks, int rank, pk_mn_table* out);
 
 /* Initial wiring of the macro nodes (initiate_mnode_wiring, src/wire.cpp:
  * 66-121, and MacroNode::setup_wiring, includes/distribute_kmers.h:550-656):

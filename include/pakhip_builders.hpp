@@ -9,10 +9,14 @@
 //       (processor(begin, end) over pairs with ->first = k-mer word,
 //       ->second = int count), std::logic_error on failure.  Also
 //       default_count_and_process_sort and concurrent_count_and_process_omp_v3
-//       (:1832-1930, the one the reference's unit test calls).  The other
-//       concurrent_* variants are left out: their results depend on the
-//       OpenMP thread count (per-thread byte or window ranges) and the
-//       omp_explicit / omp_implicit_* ones never call the processor.
+//       (:1832-1930, the one the reference's unit test calls), and the three
+//       counters src/concurrent_kmer_table_builder.cpp:97-116 calls:
+//       concurrent_count_and_process_omp_explicit (:519-666) and
+//       _omp_implicit_v0 / _v1 (:910-1022, :1363-1578).  Those three never
+//       call the processor: they return the reads and print the unique count
+//       and the instance count as the reference does (at one thread).  The
+//       remaining concurrent_* variants are left out: their results depend on
+//       the OpenMP thread count (per-thread byte or window ranges).
 //   pakhip::count_kmers / pakhip::kmer_histogram / pakhip::macro_nodes
 //       the main path (perform_kmer_counting, src/kmer_counting.cpp:3351;
 //       construct_kmer_histogram, src/kmer_histogram.cpp:72;
@@ -34,6 +38,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <cstdio>
 #include <limits>
 #include <stdexcept>
 #include <string>
@@ -138,15 +143,70 @@ class Hip_Kmer_Partial_Counter {
     return reads;
   }
 
+  // concurrent_count_and_process_omp_explicit (concurrent_kmer_counter.hpp:
+  // 519-666): the default counter's records and packed k-mers, sorted and
+  // run-length counted; the processor is not called.  Prints the number of
+  // runs ("unique kmers", the distinct k-mers; 1 when there is no k-mer, the
+  // reference's one-thread figure, which pushes one frequency for an empty
+  // range) and the instances ("concurrent_vector size").  Returns the reads.
+  template <typename processor_t>
+  size_t concurrent_count_and_process_omp_explicit(processor_t& /*processor*/) {
+    pk_table t;
+    uint64_t reads = 0;
+    const size_t n = range_bytes();
+    throw_on_error(ctx_.get(),
+                   pk_builder_count(ctx_.get(), n ? symbol_begin_ : nullptr, n, (int)klen_, read_len_,
+                                    kmer_word_traits<kmer_t>::bits, delimiter_, skip_symbol_, 0, &t, &reads),
+                   "pk_builder_count");
+    pk_timing tm;
+    throw_on_error(ctx_.get(), pk_last_timing(ctx_.get(), &tm), "pk_last_timing");
+    printf("unique kmers=%lu\n", (unsigned long)(tm.n_instances ? t.n : 1));
+    printf("concurrent_vector size=%lu\n", (unsigned long)tm.n_instances);
+    return (size_t)reads;
+  }
+
+  // concurrent_count_and_process_omp_implicit_v0 (concurrent_kmer_counter.hpp:
+  // 910-1022): every record's klen-byte windows ordered by strncmp and
+  // counted as distinct byte strings (no packing); a read cut short by the
+  // end of the range throws std::logic_error("Variable size read").  The
+  // processor is not called; prints as omp_explicit; returns the records.
+  template <typename processor_t>
+  size_t concurrent_count_and_process_omp_implicit_v0(processor_t& /*processor*/) {
+    return count_windows(PK_WINDOWS_V0);
+  }
+
+  // concurrent_count_and_process_omp_implicit_v1 (concurrent_kmer_counter.hpp:
+  // 1363-1578): as _v0, reads cut by the end of the range taken as they are
+  // (a read shorter than klen throws here; the reference's count wraps).
+  template <typename processor_t>
+  size_t concurrent_count_and_process_omp_implicit_v1(processor_t& /*processor*/) {
+    return count_windows(PK_WINDOWS_V1);
+  }
+
   pk_ctx* context() const { return ctx_.get(); }
 
  private:
+  size_t range_bytes() const {
+    return (symbol_begin_ && symbol_end_ && symbol_end_ > symbol_begin_) ? (size_t)(symbol_end_ - symbol_begin_) : 0;
+  }
+
+  size_t count_windows(int mode) {
+    if (klen_ > (size_t)std::numeric_limits<int>::max()) throw std::logic_error("klen too large");
+    const size_t n = range_bytes();
+    uint64_t reads = 0, windows = 0, distinct = 0;
+    throw_on_error(ctx_.get(),
+                   pk_builder_count_windows(ctx_.get(), n ? symbol_begin_ : nullptr, n, (int)klen_, read_len_,
+                                            delimiter_, skip_symbol_, mode, 0, &reads, &windows, &distinct),
+                   "pk_builder_count_windows");
+    printf("unique kmers=%lu\n", (unsigned long)(windows ? distinct : 1));
+    printf("concurrent_vector size=%lu\n", (unsigned long)windows);
+    return (size_t)reads;
+  }
+
   // the (k-mer, count) pairs of the counter's range, ascending; returns the
   // reads counted
   size_t count_pairs(std::vector<kmer_pair_t, alloc<kmer_pair_t>>& pairs) {
-    const size_t n = (symbol_begin_ && symbol_end_ && symbol_end_ > symbol_begin_)
-                         ? (size_t)(symbol_end_ - symbol_begin_)
-                         : 0;
+    const size_t n = range_bytes();
     if (klen_ > (size_t)std::numeric_limits<int>::max())
       throw std::logic_error("klen too large");
     pk_table t;

@@ -28,7 +28,7 @@ def build(force: bool = False) -> str:
     src = os.path.join(HERE, "pk_oracle.cpp")
     if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
         tmp = LIB_PATH + ".tmp"
-        subprocess.check_call(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", src, "-o", tmp])
+        subprocess.check_call(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-pthread", src, "-o", tmp])
         os.replace(tmp, LIB_PATH)
     return LIB_PATH
 
@@ -76,6 +76,8 @@ def lib():
         L.pko_lmer_frequency.argtypes = [vp, sz, vp]
         L.pko_lmer_owner.argtypes = [u64, i32]
         L.pko_visit_value.argtypes = [i32, i32]
+        L.pko_pipeline_ranks.argtypes = [vp, sz, i32, i32, i32, vp, vp]
+        L.pko_pipeline_ranks.restype = i32
         _LIB = L
     return _LIB
 
@@ -176,6 +178,20 @@ def macro_nodes(keys: np.ndarray, counts: np.ndarray, k: int, ref_order: bool = 
                      out["pre_order"].ctypes.data, out["suf_order"].ctypes.data)
     L.pko_nodes_free(h)
     return out
+
+
+def pipeline_ranks(fasta, k: int, t: int, nranks: int) -> dict:
+    """The reference's count -> cutoff -> macro nodes at nranks ranks, one host
+    thread per rank (the CPU baseline): distinct / kept / nodes / instances
+    totals over the ranks, min_bucket, hist."""
+    a, n = _buf(fasta)
+    stats = np.zeros(5, dtype=np.uint64)
+    hist = np.zeros(max(t, 1), dtype=np.uint64)
+    rc = lib().pko_pipeline_ranks(a.ctypes.data, n, k, t, nranks, stats.ctypes.data, hist.ctypes.data)
+    if rc and rc != PK_EEMPTY:
+        raise OracleError(rc, "pko_pipeline_ranks")
+    return {"distinct": int(stats[0]), "kept": int(stats[1]), "nodes": int(stats[2]),
+            "instances": int(stats[3]), "min_bucket": int(stats[4]), "hist": hist, "rc": rc}
 
 
 def lmer_frequency(fasta) -> np.ndarray:

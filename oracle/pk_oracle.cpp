@@ -30,6 +30,7 @@
 #include <cstring>
 #include <map>
 #include <new>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -501,6 +502,156 @@ struct Nodes {
 
 bool valid_k(int k) { return k >= 8 && k <= 64; }
 
+// ---------------------------------------------------------------------------
+// The reference at P ranks, one host thread per rank (the CPU baseline of
+// the benchmark: mpiexec -np P with OMP_NUM_THREADS=1, as the reference's own
+// runs, scripts/cori/human_genome/run_human_str.sh:20,41).  Every phase is a
+// parallel step over the ranks; joining the threads stands for the
+// collective between phases.
+//   1. DivideReads (src/input_reading.cpp:69-243): rank r reads from the first
+//      '>' at or after r * (n / P) to the first '>' at or after (r+1) * (n / P).
+//   2. Sliding_window_l on the share, summed over ranks (MPI_Allreduce,
+//      src/kmer_counting.cpp:391-438, 3366).
+//   3. Sliding_window (:2890-3154): each window goes to rank hash31(min L-mer)
+//      % P; per destination std::sort + run-length count (:3057-3086,
+//      3156-3261; one flush per rank).
+//   4. transfer_kmers (:254-352): rank r sorts the pairs it receives and
+//      aggregates them into its table.
+//   5. construct_kmer_histogram (src/kmer_histogram.cpp:72-249): histogram
+//      summed over ranks, min_bucket, each kept k-mer sent to the owners of
+//      its pred and succ (retrieve_proc_id of kmer_t: hash31 for 64-bit keys,
+//      uhash31 for 128-bit).
+//   6. construct_macro_nodes (src/mn_construct.cpp:212-477): rank r builds the
+//      nodes whose keys it owns from what it received.
+// Totals: distinct k-mers, kept k-mers and nodes over the ranks (equal to the
+// one-rank figures; tested), and the histogram.
+
+inline long long uhash31_128(u128 x) {
+  u128 r = (u128)HASHA * x + (u128)HASHB;
+  r = ((r >> 31) + r) & (u128)2147483647;
+  return (long long)r;
+}
+inline int node_owner(uint64_t x, int P) { return (int)((int)hash31((long long)HASHA, (long long)HASHB, (long long)x) % P); }
+inline int node_owner(u128 x, int P) { return (int)((int)uhash31_128(x) % P); }
+
+template <class F>
+void for_ranks(int P, F&& f) {
+  std::vector<std::thread> th;
+  for (int r = 1; r < P; r++) th.emplace_back(f, r);
+  f(0);
+  for (auto& t : th) t.join();
+}
+
+struct RankRun {
+  uint64_t distinct = 0, kept = 0, nodes = 0, instances = 0;
+  int min_bucket = 0;
+  std::vector<uint64_t> hist;
+};
+
+template <class K>
+int pipeline_ranks(const char* fa, size_t len, int k, int t, int P, RankRun& out) {
+  Enc<K> e(k);
+  // 1. shares
+  std::vector<size_t> lo(P), hi(P);
+  const size_t step = len / (size_t)P;
+  auto snap = [&](size_t pos) {
+    while (pos < len && fa[pos] != '>') pos++;
+    return pos;
+  };
+  for (int r = 0; r < P; r++) {
+    lo[r] = r == 0 ? 0 : snap((size_t)r * step);
+    hi[r] = r == P - 1 ? len : snap((size_t)(r + 1) * step);
+    if (hi[r] < lo[r]) hi[r] = lo[r];
+  }
+  // 2. L-mer frequencies
+  std::vector<std::vector<uint64_t>> lf(P);
+  std::vector<int> rcs(P, E_OK);
+  for_ranks(P, [&](int r) { rcs[r] = lmer_frequency(fa + lo[r], hi[r] - lo[r], lf[r]); });
+  for (int r = 0; r < P; r++)
+    if (rcs[r]) return rcs[r];
+  std::vector<uint64_t> freq(lf[0].size(), 0);
+  for (int r = 0; r < P; r++)
+    for (size_t i = 0; i < freq.size(); i++) freq[i] += lf[r][i];
+  // 3. route, sort and run-length count per destination
+  std::vector<std::vector<std::vector<Pair<K>>>> sent(P, std::vector<std::vector<Pair<K>>>(P));
+  std::vector<uint64_t> inst(P, 0);
+  for_ranks(P, [&](int r) {
+    std::vector<std::vector<K>> part(P);
+    rcs[r] = sliding_window<K>(fa + lo[r], hi[r] - lo[r], e, P, P > 1 ? &freq : nullptr,
+                               [&](K km, int owner) { part[owner].push_back(km); inst[r]++; });
+    for (int d = 0; d < P; d++) {
+      auto& v = part[d];
+      std::sort(v.begin(), v.end());
+      auto& o = sent[r][d];
+      for (size_t i = 0; i < v.size();) {
+        size_t j = i;
+        while (j < v.size() && v[j] == v[i]) j++;
+        o.push_back(Pair<K>{v[i], (int)(j - i)});
+        i = j;
+      }
+      std::vector<K>().swap(v);
+    }
+  });
+  for (int r = 0; r < P; r++)
+    if (rcs[r]) return rcs[r];
+  // 4. receive, sort, aggregate
+  std::vector<std::vector<Pair<K>>> table(P);
+  auto by_seq = [](const Pair<K>& a, const Pair<K>& b) { return a.seq < b.seq; };
+  for_ranks(P, [&](int r) {
+    auto& tb = table[r];
+    for (int s = 0; s < P; s++) {
+      tb.insert(tb.end(), sent[s][r].begin(), sent[s][r].end());
+      std::vector<Pair<K>>().swap(sent[s][r]);
+    }
+    std::sort(tb.begin(), tb.end(), by_seq);
+    aggregate_inplace(tb);
+  });
+  // 5. histogram, min_bucket, kept k-mers to their node owners
+  out.hist.assign((size_t)t, 0);
+  std::vector<std::vector<uint64_t>> h(P, std::vector<uint64_t>((size_t)t, 0));
+  for_ranks(P, [&](int r) {
+    for (auto& pr : table[r])
+      if (pr.cnt >= 0 && pr.cnt < t) h[r][pr.cnt]++;
+  });
+  for (int r = 0; r < P; r++)
+    for (int c = 0; c < t; c++) out.hist[c] += h[r][c];
+  int mb = t;
+  if (t > 1) {
+    mb = 1;
+    for (int i = 2; i < t; i++)
+      if (out.hist[i] < out.hist[mb]) mb = i;
+  }
+  out.min_bucket = mb;
+  std::vector<std::vector<std::vector<Pair<K>>>> mn(P, std::vector<std::vector<Pair<K>>>(P));
+  std::vector<uint64_t> kept(P, 0);
+  for_ranks(P, [&](int r) {
+    for (auto& pr : table[r]) {
+      if (pr.cnt < mb) continue;
+      kept[r]++;
+      const int hp = node_owner(e.pred(pr.seq), P), hs = node_owner(e.succ(pr.seq), P);
+      mn[r][hp].push_back(pr);
+      if (hs != hp) mn[r][hs].push_back(pr);
+    }
+  });
+  // 6. macro nodes of the owned keys
+  std::vector<uint64_t> nn(P, 0);
+  for_ranks(P, [&](int r) {
+    std::vector<Pair<K>> recv;
+    for (int s = 0; s < P; s++) recv.insert(recv.end(), mn[s][r].begin(), mn[s][r].end());
+    std::vector<Node<K>> nodes;
+    if (!recv.empty()) macro_nodes<K>(recv.data(), recv.size(), k, 1, nodes);
+    for (auto& nd : nodes)
+      if (node_owner(nd.key, P) == r) nn[r]++;
+  });
+  for (int r = 0; r < P; r++) {
+    out.distinct += table[r].size();
+    out.kept += kept[r];
+    out.nodes += nn[r];
+    out.instances += inst[r];
+  }
+  return E_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -641,6 +792,25 @@ int pko_macro_nodes(const uint64_t* keys, const int32_t* counts, size_t V, int k
   }
   *out = (pko_nodes*)nd;
   return E_OK;
+}
+
+// The reference pipeline (count, cutoff, macro nodes) at nranks ranks, one
+// host thread per rank (pipeline_ranks above).  stats: distinct, kept, nodes,
+// instances, min_bucket; hist: t entries.  Returns E_EMPTY when nothing is
+// kept (the reference aborts there).
+int pko_pipeline_ranks(const char* fasta, size_t len, int k, int t, int nranks, uint64_t* stats, uint64_t* hist) {
+  if (!valid_k(k) || nranks < 1 || nranks > 1024 || t < 1) return E_INVAL;
+  RankRun rr;
+  const int rc = k <= 32 ? pipeline_ranks<uint64_t>(fasta, len, k, t, nranks, rr)
+                         : pipeline_ranks<u128>(fasta, len, k, t, nranks, rr);
+  if (rc) return rc;
+  stats[0] = rr.distinct;
+  stats[1] = rr.kept;
+  stats[2] = rr.nodes;
+  stats[3] = rr.instances;
+  stats[4] = (uint64_t)rr.min_bucket;
+  for (int i = 0; i < t; i++) hist[i] = rr.hist[i];
+  return rr.kept ? E_OK : E_EMPTY;
 }
 
 size_t pko_nodes_size(const pko_nodes* p) { return ((const Nodes*)p)->n_pre.size(); }

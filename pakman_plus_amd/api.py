@@ -89,8 +89,8 @@ EXPORTS = ("pk_ctx_create", "pk_ctx_destroy", "pk_last_error", "pk_ctx_stream",
            "pk_ctx_device_bytes", "pk_ctx_set_tuning", "pk_ctx_release_buffers", "pk_count",
            "pk_count_noalloc",
            "pk_count_format", "pk_normalize", "pk_download_normalized",
-           "pk_count_rank_share",
-           "pk_lmer_frequency", "pk_builder_count",
+           "pk_count_rank_share", "pk_count_rank_share_by_key", "pk_macro_nodes_rank_share",
+           "pk_lmer_frequency", "pk_builder_count", "pk_builder_count_windows",
            "pk_cutoff", "pk_cutoff_threshold", "pk_kept_table", "pk_macro_nodes", "pk_mn_wiring", "pk_download_table", "pk_download_kept",
            "pk_download_nodes", "pk_last_timing", "pk_server_create", "pk_server_destroy",
            "pk_server_now", "pk_server_submit", "pk_server_submit_batch", "pk_server_wait",
@@ -141,11 +141,18 @@ def load_library():
     L.pk_download_normalized.restype = i32
     L.pk_count_rank_share.argtypes = [vp, vp, u64, i32, i32, i32, i32, ctypes.POINTER(_Table)]
     L.pk_count_rank_share.restype = i32
+    L.pk_count_rank_share_by_key.argtypes = [vp, vp, u64, i32, i32, i32, i32, ctypes.POINTER(_Table)]
+    L.pk_count_rank_share_by_key.restype = i32
+    L.pk_macro_nodes_rank_share.argtypes = [vp, i32, i32, ctypes.POINTER(_MnTable)]
+    L.pk_macro_nodes_rank_share.restype = i32
     L.pk_lmer_frequency.argtypes = [vp, vp, u64, i32, vp]
     L.pk_lmer_frequency.restype = i32
     L.pk_builder_count.argtypes = [vp, vp, u64, i32, u64, i32, ctypes.c_char, ctypes.c_char, i32,
                                    ctypes.POINTER(_Table), ctypes.POINTER(u64)]
     L.pk_builder_count.restype = i32
+    L.pk_builder_count_windows.argtypes = [vp, vp, u64, i32, u64, ctypes.c_char, ctypes.c_char, i32, i32,
+                                           ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64)]
+    L.pk_builder_count_windows.restype = i32
     L.pk_cutoff.argtypes = [vp, i32, vp, ctypes.POINTER(i32), ctypes.POINTER(u64)]
     L.pk_cutoff.restype = i32
     L.pk_cutoff_threshold.argtypes = [vp, u64, ctypes.POINTER(u64)]
@@ -339,6 +346,26 @@ class Context:
         self.table = TableView(int(t.n), int(t.words), int(t.k), t.keys or 0, t.counts or 0)
         return self.table
 
+    def count_rank_share_by_key(self, fasta, k: int, nranks: int, rank: int) -> TableView:
+        """Rank `rank`'s share at `nranks` ranks with the -p counters' owner
+        rule: retrieve_proc_id(kmer) (hash31 / uhash31 of the k-mer) % nranks."""
+        t = _Table()
+        a = _as_u8(fasta)
+        self._check(self._lib.pk_count_rank_share_by_key(self._h, a.ctypes.data, a.nbytes, k, nranks, rank, 0,
+                                                         ctypes.byref(t)))
+        self.k = k
+        self.table = TableView(int(t.n), int(t.words), int(t.k), t.keys or 0, t.counts or 0)
+        return self.table
+
+    def macro_nodes_rank_share(self, nranks: int, rank: int) -> int:
+        """The macro nodes rank `rank` holds at `nranks` ranks (node keys it
+        owns by retrieve_proc_id); the context's node table becomes the share."""
+        m = _MnTable()
+        self._check(self._lib.pk_macro_nodes_rank_share(self._h, nranks, rank, ctypes.byref(m)))
+        self.n_nodes = int(m.n)
+        self.nodes_ptr = m.nodes or 0
+        return self.n_nodes
+
     def lmer_frequency(self, fasta) -> np.ndarray:
         """4^8 L-mer frequencies (Sliding_window_l, kmer_counting.cpp:391-438)."""
         a = _as_u8(fasta)
@@ -367,6 +394,20 @@ class Context:
         self.k = klen
         self.table = TableView(int(t.n), int(t.words), int(t.k), t.keys or 0, t.counts or 0)
         return int(nr.value)
+
+    def builder_count_windows(self, fasta, klen: int, read_len: int = 100, mode: str = "v0",
+                              delimiter: bytes = b"\n", skip_symbol: bytes = b">") -> tuple[int, int, int]:
+        """Window count of the implicit builder counters (omp_implicit_v0 /
+        _v1, concurrent_kmer_counter.hpp:910-1022,1363-1578): (records,
+        windows, distinct klen-byte windows under strncmp)."""
+        a = _as_u8(fasta)
+        r, w, d = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        m = {"v0": 0, "v1": 1}[mode]
+        self._check(self._lib.pk_builder_count_windows(self._h, a.ctypes.data, a.nbytes, klen, read_len, delimiter,
+                                                       skip_symbol, m, 0, ctypes.byref(r), ctypes.byref(w),
+                                                       ctypes.byref(d)))
+        self.table = None
+        return int(r.value), int(w.value), int(d.value)
 
     def table_host(self):
         """(keys[n, words] uint64, counts[n] int32) of the current table."""

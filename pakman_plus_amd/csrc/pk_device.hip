@@ -542,6 +542,93 @@ __global__ void k_bld_write(const uint8_t* fa, uint64_t n, const ull* ev, uint64
   kc[r] = (uint32_t)(rl - (uint64_t)klen + 1);
 }
 
+// Window records of the implicit builder counters
+// (Concurrent_Kmer_Partial_Counter::concurrent_count_and_process_omp_implicit_v0
+// / _v1, src/concurrent_kmer_counter.hpp:910-1022,1363-1578): the records of
+// bld_record (same scan), every one counted, whatever its length.  flag[e] =
+// event e starts a record; bad[0] = a read shorter than read_len (cut at the
+// end of the buffer), bad[1] = a read shorter than klen.
+__global__ void k_win_flags(const uint8_t* fa, uint64_t n, const ull* ev, uint64_t m, int klen,
+                            uint64_t read_len, BldSyms sy, uint32_t* flag, uint32_t* bad) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  uint64_t rb, rl;
+  const bool rec = bld_record(fa, n, ev, m, e, 0, read_len, sy, &rb, &rl);
+  flag[e] = rec ? 1u : 0u;
+  if (rec && rl != read_len) atomicOr(&bad[0], 1u);
+  if (rec && rl < (uint64_t)klen) atomicOr(&bad[1], 1u);
+}
+
+// Writes the window records: start, length and windows (len - klen + 1, or 0).
+__global__ void k_win_write(const uint8_t* fa, uint64_t n, const ull* ev, uint64_t m, int klen,
+                            uint64_t read_len, BldSyms sy, const ull* idx, ull* rstart, uint32_t* rlen,
+                            uint32_t* wc) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  uint64_t rb, rl;
+  if (!bld_record(fa, n, ev, m, e, 0, read_len, sy, &rb, &rl)) return;
+  const ull r = idx[e];
+  rstart[r] = rb;
+  rlen[r] = (uint32_t)rl;
+  wc[r] = rl >= (uint64_t)klen ? (uint32_t)(rl - (uint64_t)klen + 1) : 0u;
+}
+
+// The byte values present in the reads: bit v of present[v / 32].
+__global__ void k_win_alpha(const uint8_t* fa, const ull* rstart, const uint32_t* rlen, uint64_t R,
+                            uint32_t* present) {
+  __shared__ uint32_t p[8];
+  if (threadIdx.x < 8) p[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.y + threadIdx.y; r < R; r += (uint64_t)gridDim.x * blockDim.y) {
+    const ull s = rstart[r];
+    const uint32_t len = rlen[r];
+    for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
+      const uint32_t v = fa[s + i];
+      atomicOr(&p[v >> 5], 1u << (v & 31));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 8 && threadIdx.y == 0 && p[threadIdx.x]) atomicOr(&present[threadIdx.x], p[threadIdx.x]);
+}
+
+// Window keys: the klen bytes of each window as codes of `bits` bits (code =
+// rank of the byte value among the values present, so equal windows, and
+// only those, get equal keys), first byte highest.  strncmp compares up to the
+// first NUL: with NUL present (code 0) every code after a window's first NUL
+// is 0.  One thread per read, windows in read order at woff[r].
+template <class Key>
+__global__ void k_win_keys(const uint8_t* fa, const ull* rstart, const uint32_t* rlen, const ull* woff,
+                           uint64_t R, int klen, int bits, const uint8_t* lut, int has_nul, Key* S) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const ull s = rstart[r];
+  const uint32_t len = rlen[r];
+  if (len < (uint32_t)klen) return;
+  const uint32_t nw = len - (uint32_t)klen + 1;
+  const int kb = klen * bits;
+  const Key mask = kb >= (int)(8 * sizeof(Key)) ? ~Key(0) : ((Key(1) << kb) - 1);
+  Key* out = S + woff[r];
+  if (!has_nul) {
+    Key km = 0;
+    for (uint32_t i = 0; i < len; i++) {
+      km = ((km << bits) | (Key)lut[fa[s + i]]) & mask;
+      if (i + 1 >= (uint32_t)klen) out[i + 1 - (uint32_t)klen] = km;
+    }
+    return;
+  }
+  for (uint32_t w = 0; w < nw; w++) {
+    Key km = 0;
+    bool nul = false;
+    for (int j = 0; j < klen; j++) {
+      const uint8_t v = fa[s + w + j];
+      const uint32_t code = nul ? 0u : (uint32_t)lut[v];
+      nul = nul || v == 0;
+      km = (km << bits) | (Key)code;
+    }
+    out[w] = km;
+  }
+}
+
 // Records (kmer_counting.cpp:2920-2945).  Candidate r uses header newline
 // nl[j0+2r] and sequence newline nl[j0+2r+1] (or the end of the buffer).
 __global__ void k_reads(const uint8_t* fa, uint64_t n, const ull* nl, uint64_t m, uint64_t p0,
@@ -756,8 +843,9 @@ constexpr int LMER_L = 8;
 constexpr uint32_t LMER_BINS = 1u << (2 * LMER_L);
 
 struct OwnCfg {
-  const ull* freq;  // LMER_BINS L-mer frequencies
+  const ull* freq;  // LMER_BINS L-mer frequencies (min-L-mer owner)
   int P, rank;
+  int by_key;       // owner of the whole k-mer instead (the -p counters)
 };
 
 struct OwnState {
@@ -765,6 +853,22 @@ struct OwnState {
   ull min_f;
   int min_pos;
 };
+
+// retrieve_proc_id of a k-mer word (src/distribute_kmers.cpp:95-120): hash31
+// (includes/distribute_kmers.h:856-873: signed 64-bit a * x + b, arithmetic
+// shift) when kmer_t is 64 bits, uhash31 (:836-854: the same in unsigned
+// 128-bit arithmetic) when it is 128 bits; then (int) % P.  Both wrap as the
+// reference's builds do.
+__device__ __forceinline__ int key_owner(uint64_t x, int P) {
+  const long long v = (long long)(68111ull * x + 105929ull);
+  const long long h = (long long)((ull)(v >> 31) + (ull)v) & 2147483647LL;
+  return (int)h % P;
+}
+__device__ __forceinline__ int key_owner(u128 x, int P) {
+  u128 r = (u128)68111 * x + (u128)105929;
+  r = ((r >> 31) + r) & (u128)2147483647;
+  return (int)(long long)r % P;
+}
 
 __device__ __forceinline__ int lmer_owner(uint32_t lmer, int P) {
   const long long v = 68111LL * (long long)lmer + 105929LL;  // hasha * x + hashb
@@ -778,6 +882,9 @@ template <class Key, bool OWN>
 __device__ __forceinline__ bool own_step(OwnState& o, const OwnCfg& cf, Key km, uint32_t el, uint32_t idx,
                                          int K) {
   if (!OWN) return true;
+  // the -p counters own a k-mer by a hash of the whole word
+  // (kmer_counting.cpp:1949-1950)
+  if (cf.by_key) return idx >= (uint32_t)(K - 1) && key_owner(km, cf.P) == cf.rank;
   o.lm = ((o.lm << 2) | el) & (LMER_BINS - 1);
   if (idx < (uint32_t)(K - 1)) {
     if (idx >= (uint32_t)(LMER_L - 1)) {
@@ -1065,9 +1172,12 @@ __global__ __launch_bounds__(MS_NT) void k_ms_scatter(const Key* Sin, const ull*
   }
   const int np = m > (int)threadIdx.x ? (m - (int)threadIdx.x - 1) / MS_NT + 1 : 0;
   __syncthreads();
+  // the histogram's atomic returns each key's rank inside its digit (two
+  // 16-bit ranks per register), so the sort needs no second atomic pass
+  uint32_t rk[(Q + 1) / 2] = {};
 #pragma unroll
   for (int j = 0; j < Q; j++)
-    if (j < np) atomicAdd(&L.h[key_digit<Key>(kr[j], dpos, dbits)], 1u);
+    if (j < np) rk[j / 2] |= atomicAdd(&L.h[key_digit<Key>(kr[j], dpos, dbits)], 1u) << (16 * (j & 1));
   __syncthreads();
   const uint32_t v = (int)threadIdx.x < ns ? L.h[threadIdx.x] : 0u;
   const ull ex = block_excl_scan<MS_NT>(v);
@@ -1080,8 +1190,7 @@ __global__ __launch_bounds__(MS_NT) void k_ms_scatter(const Key* Sin, const ull*
   for (int j = 0; j < Q; j++) {
     if (j < np) {
       const uint32_t d = key_digit<Key>(kr[j], dpos, dbits);
-      const uint32_t pos = atomicAdd(&L.tcur[d], 1u);
-      L.T1[pos] = kr[j];
+      L.T1[L.tcur[d] + ((rk[j / 2] >> (16 * (j & 1))) & 0xFFFFu)] = kr[j];
     }
   }
   __syncthreads();
@@ -1092,10 +1201,14 @@ __global__ __launch_bounds__(MS_NT) void k_ms_scatter(const Key* Sin, const ull*
 }
 
 // Persistent form for long levels: block i takes the tiles
-// [i * per, (i + 1) * per) in order and loads the next tile's keys and
-// output bases while it sorts the current one (at most 128 VGPRs: two blocks
-// of 8 waves per CU).  Consecutive tiles of a bucket write adjacent runs of
-// each sub-digit, so one CU completes the lines at their joins.
+// [i * per, (i + 1) * per) in order and loads the next tile's keys while it
+// sorts the current one (at most 128 VGPRs: two blocks of 8 waves per CU).
+// Consecutive tiles of a bucket write adjacent runs of each sub-digit, so one
+// CU completes the lines at their joins.  The output base of sub-digit s for
+// tile t + 1 of a bucket is tile t's base plus tile t's count of s (the scan
+// of H in (bucket, digit, tile) order), so bases are read from O only for a
+// block's first tile in each bucket: one line per digit per bucket instead of
+// per tile.
 template <class Key>
 __global__ __launch_bounds__(MS_NT, 4) void k_ms_scatter_p(const Key* Sin, const ull* boff, const ull* tp,
                                                            int nb, int dpos, int dbits, const ull* O,
@@ -1134,16 +1247,21 @@ __global__ __launch_bounds__(MS_NT, 4) void k_ms_scatter_p(const Key* Sin, const
         const int idx = tid + j * MS_NT;
         kn[j] = idx < mn ? Sin[b0n + idx] : Key(0);
       }
-      if (tid < ns) bn = O[ms_index(tp, an, tid, tn, dbits)];
+      // a new bucket: its first base from the scan
+      if (tid < ns && an != a) bn = O[ms_index(tp, an, tid, tn, dbits)];
     }
     if (tid < ns) L.h[tid] = 0;
     const int np = m > tid ? (m - tid - 1) / MS_NT + 1 : 0;
     __syncthreads();
+    // ranks inside each digit from the histogram's atomics (see k_ms_scatter)
+    uint32_t rk[(Q + 1) / 2] = {};
 #pragma unroll
     for (int j = 0; j < Q; j++)
-      if (j < np) atomicAdd(&L.h[key_digit<Key>(kr[j], dpos, dbits)], 1u);
+      if (j < np) rk[j / 2] |= atomicAdd(&L.h[key_digit<Key>(kr[j], dpos, dbits)], 1u) << (16 * (j & 1));
     __syncthreads();
     const uint32_t v = tid < ns ? L.h[tid] : 0u;
+    // the next tile of the same bucket starts its runs where this one's end
+    if (an == a) bn = bs + v;
     const ull ex = block_excl_scan<MS_NT>(v);
     if (tid < ns) {
       L.tcur[tid] = (uint32_t)ex;
@@ -1154,8 +1272,7 @@ __global__ __launch_bounds__(MS_NT, 4) void k_ms_scatter_p(const Key* Sin, const
     for (int j = 0; j < Q; j++) {
       if (j < np) {
         const uint32_t d = key_digit<Key>(kr[j], dpos, dbits);
-        const uint32_t pos = atomicAdd(&L.tcur[d], 1u);
-        L.T1[pos] = kr[j];
+        L.T1[L.tcur[d] + ((rk[j / 2] >> (16 * (j & 1))) & 0xFFFFu)] = kr[j];
       }
     }
     __syncthreads();
@@ -2269,6 +2386,27 @@ __global__ void k_mn_compact(const MnRec* NT, const MnPlan* plan, const ull* ng,
   for (ull i = threadIdx.x; i < 3 * c; i += blockDim.x) dst[i] = src[i];
 }
 
+// Macro-node share of rank `rank` at P ranks: nodes whose (k-1)-mer key is
+// owned by the rank (retrieve_proc_id(pred / succ), mn_construct.cpp:
+// 234-247; the rank receives every kept k-mer of its nodes,
+// kmer_histogram.cpp:140-170, so its records are the one-rank records).
+template <class Key>
+__global__ void k_node_share_flags(const MnRec* nodes, uint64_t M, int P, int rank, uint32_t* flag) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  Key key;
+  if (sizeof(Key) == 8) key = (Key)nodes[i].key_lo;
+  else key = (Key)(((u128)nodes[i].key_hi << 64) | (u128)nodes[i].key_lo);
+  flag[i] = key_owner(key, P) == rank ? 1u : 0u;
+}
+
+__global__ void k_node_share_write(const MnRec* nodes, uint64_t M, const uint32_t* flag, const ull* pos,
+                                   MnRec* out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M || !flag[i]) return;
+  out[pos[i]] = nodes[i];
+}
+
 // ---------------------------------------------------------------------------
 // Host side.
 
@@ -2325,6 +2463,7 @@ struct pk_ctx {
   char* arena = nullptr;
   uint64_t arena_cap = 0, arena_used = 0;
   bool arena_fixed[B_COUNT] = {};
+  uint8_t win_lut[256] = {};  // byte -> code of the last pk_builder_count_windows
 
   int fail(int code, const std::string& what) {
     err = what;
@@ -2759,114 +2898,20 @@ int records_builder(pk_ctx* c, const uint8_t* fa, uint64_t n, int klen, uint64_t
   return PK_OK;
 }
 
-// Count the k-mers of the read table (rstart, rlen)[0..R) (records in file
-// order), N instances.
-// K = window length, KB = key bits (2K on the main path, the whole word for
-// the builder encoding), mask = word mask of the rolling update.
-template <class Key, int ENC, bool OWN = false>
-int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, const uint32_t* rlen,
-                  uint64_t R, uint64_t N, int K, int KB, Key mask, OwnCfg cf = OwnCfg{nullptr, 1, 0}) {
-  constexpr int CAP = HashCfg<Key>::LIMIT;
+// Further partition levels from memory, the per-bucket count and the dense
+// table.  S holds the N keys (KB key bits) split into B buckets by their top
+// `fixed` bits, at offsets boff (the B_BOFF buffer); levels of <= MS_MAX_BITS
+// follow until btot bits are fixed.  S2 is the other instance array, C the
+// per-instance counts; small[4] (the overflow count) starts at zero.
+template <class Key>
+int count_partitioned(pk_ctx* c, Key* S, Key* S2, int32_t* C, ull* boff, uint64_t B, int fixed, int btot,
+                      int KB, uint64_t N) {
+  Key* TK;
+  int32_t* TC;
   hipStream_t st = c->stream;
+  const pk_tuning& tun = c->tuning;
   ull* small;
   PK_TRY(need(c, B_SMALL, 16, &small));
-  c->R = R;
-  c->N = N;
-  c->k = K;
-  c->words = sizeof(Key) / 8;
-  c->timing.n_instances = N;
-  c->timing.n_reads = R;
-  c->ms_levels = 0;
-  PK_HIP(c, hipEventRecord(c->ev[1], st));
-  Key *S, *TK;
-  int32_t *C, *TC;
-  if (N == 0) {
-    c->U = 0;
-    PK_TRY(need(c, B_TK, 1, &TK));
-    PK_TRY(need(c, B_TC, 1, &TC));
-    c->have_table = true;
-    return PK_OK;
-  }
-  // 3. partition: level 1 fused with the parse, further levels from memory,
-  // at most 8 bits per level, until a bucket holds ~1.5 hash tables' worth
-  // of instances.  Write bases come from scanned digit-major histograms.
-  // with owner routing, about N / P instances are this rank's
-  const pk_tuning& tun = c->tuning;
-  const float bf = tun.bucket_factor > 0.f ? tun.bucket_factor : 3.0f;
-  const int btot = choose_bits(OWN ? (N + cf.P - 1) / cf.P : N, KB / 2, CAP, bf);
-  int d1;
-  if (tun.l1_bits > 0 && tun.l1_bits <= L1_MAX_BITS) {
-    d1 = btot < tun.l1_bits ? btot : tun.l1_bits;
-  } else if (btot > L1_DEFAULT_BITS + MS_MAX_BITS) {
-    // three or more levels anyway: the narrowest level-1 digit (>= 5 bits)
-    // that adds no level, since every level-1 digit keeps a partly written
-    // line open per block (k = 64, 2.9e9 instances, 2^21 buckets: level-1
-    // scatter 33.8 ms at 8 bits, 13.8 ms at 5)
-    const int further = (btot - L1_DEFAULT_BITS + MS_MAX_BITS - 1) / MS_MAX_BITS;
-    d1 = std::max(5, btot - further * MS_MAX_BITS);
-  } else {
-    d1 = btot < L1_DEFAULT_BITS ? btot : L1_DEFAULT_BITS;
-  }
-  const uint64_t B1 = 1ull << d1;
-  // reads per block: up to 8 chunks of L1_NT, fewer for small inputs so that
-  // the grid still spreads over the CUs
-  const int rpb = L1_NT * (tun.l1_chunks > 0 && tun.l1_chunks <= 64
-                               ? tun.l1_chunks
-                               : (int)std::min<uint64_t>(8, std::max<uint64_t>(1, R / (L1_NT * 1024ull))));
-  const uint64_t nblk = (R + rpb - 1) / rpb;
-  // LDS staging: room for one record per thread of the input's mean size,
-  // + 25 %, for each of the two level-1 kernels
-  auto staging = [&](int nt, int cap) {
-    const double want = (double)nt * ((double)n / (double)R) * 1.25;
-    const int b = (int)std::min<double>(cap, std::max<double>(L1_SB_MIN, want));
-    return (b + 1023) & ~1023;
-  };
-  const int sb = staging(L1_NT, L1_SB_MAX);
-  constexpr int snt = L1Scatter<Key, ENC, OWN>::NT;
-  const int sbs = staging(snt, L1Scatter<Key, ENC, OWN>::SB_MAX);
-  ull *boff, *O;
-  uint32_t* H;
-  Key* S2;
-  PK_TRY(need(c, B_BOFF, B1 + 1, &boff));
-  PK_TRY(need(c, B_HIST1, B1 * nblk, &H));
-  PK_TRY(need(c, B_OFF1, B1 * nblk + 1, &O));
-  PK_TRY(need(c, B_S, N, &S));
-  PK_TRY(need(c, B_S2, N, &S2));
-  PK_TRY(need(c, B_C, N, &C));
-  c->timing.bucket_bits = btot;
-  PK_HIP(c, hipEventRecord(c->ev[2], st));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_hist<Key, ENC, OWN>), dim3((unsigned)nblk), dim3(L1_NT),
-                     l1_lds_bytes(sb, (int)B1, 4), st, fa, n, rstart, rlen, R, K, KB, mask, d1, rpb, sb, cf, H);
-  PK_HIP(c, hipGetLastError());
-  PK_HIP(c, hipEventRecord(c->ev[3], st));
-  PK_TRY(scan_excl<uint32_t>(c, H, O, B1 * nblk));
-  if (OWN) {
-    // this rank's instances
-    ull no = 0;
-    PK_TRY(d2h(c, &no, O + B1 * nblk, 8));
-    N = no;
-    c->N = N;
-    c->timing.n_instances = N;
-    if (N == 0) {
-      c->U = 0;
-      PK_TRY(need(c, B_TK, 1, &TK));
-      PK_TRY(need(c, B_TC, 1, &TC));
-      c->have_table = true;
-      return PK_OK;
-    }
-  }
-  hipLaunchKernelGGL(k_l1_boff, dim3(grid_for(B1 + 1, 256)), dim3(256), 0, st, O, (int)B1, nblk, (ull)N,
-                     boff);
-  PK_HIP(c, hipGetLastError());
-  PK_HIP(c, hipEventRecord(c->ev[4], st));
-  constexpr int sgf = L1Scatter<Key, ENC, OWN>::GF;
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key, ENC, OWN>), dim3((unsigned)((nblk + sgf - 1) / sgf)),
-                     dim3(snt), l1_lds_bytes(sbs, (int)B1, 12), st, fa, n, rstart, rlen, R, K, KB, mask, d1, rpb,
-                     sbs, cf, O, nblk, S);
-  PK_HIP(c, hipGetLastError());
-  PK_HIP(c, hipEventRecord(c->ev[16], st));
-  uint64_t B = B1;
-  int fixed = d1;
   int set = 0;
   Key* Sin = S;
   Key* Sout = S2;
@@ -3025,6 +3070,163 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   return PK_OK;
 }
 
+// Count the k-mers of the read table (rstart, rlen)[0..R) (records in file
+// order), N instances.
+// K = window length, KB = key bits (2K on the main path, the whole word for
+// the builder encoding), mask = word mask of the rolling update.
+template <class Key, int ENC, bool OWN = false>
+int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, const uint32_t* rlen,
+                  uint64_t R, uint64_t N, int K, int KB, Key mask, OwnCfg cf = OwnCfg{nullptr, 1, 0, 0}) {
+  constexpr int CAP = HashCfg<Key>::LIMIT;
+  hipStream_t st = c->stream;
+  ull* small;
+  PK_TRY(need(c, B_SMALL, 16, &small));
+  c->R = R;
+  c->N = N;
+  c->k = K;
+  c->words = sizeof(Key) / 8;
+  c->timing.n_instances = N;
+  c->timing.n_reads = R;
+  c->ms_levels = 0;
+  PK_HIP(c, hipEventRecord(c->ev[1], st));
+  Key *S, *TK;
+  int32_t *C, *TC;
+  if (N == 0) {
+    c->U = 0;
+    PK_TRY(need(c, B_TK, 1, &TK));
+    PK_TRY(need(c, B_TC, 1, &TC));
+    c->have_table = true;
+    return PK_OK;
+  }
+  // 3. partition: level 1 fused with the parse, further levels from memory,
+  // at most 8 bits per level, until a bucket holds ~1.5 hash tables' worth
+  // of instances.  Write bases come from scanned digit-major histograms.
+  // with owner routing, about N / P instances are this rank's
+  const pk_tuning& tun = c->tuning;
+  const float bf = tun.bucket_factor > 0.f ? tun.bucket_factor : 3.0f;
+  const int btot = choose_bits(OWN ? (N + cf.P - 1) / cf.P : N, KB / 2, CAP, bf);
+  int d1;
+  if (tun.l1_bits > 0 && tun.l1_bits <= L1_MAX_BITS) {
+    d1 = btot < tun.l1_bits ? btot : tun.l1_bits;
+  } else if (btot > L1_DEFAULT_BITS + MS_MAX_BITS) {
+    // three or more levels anyway: the narrowest level-1 digit (>= 5 bits)
+    // that adds no level, since every level-1 digit keeps a partly written
+    // line open per block (k = 64, 2.9e9 instances, 2^21 buckets: level-1
+    // scatter 33.8 ms at 8 bits, 13.8 ms at 5)
+    const int further = (btot - L1_DEFAULT_BITS + MS_MAX_BITS - 1) / MS_MAX_BITS;
+    d1 = std::max(5, btot - further * MS_MAX_BITS);
+  } else {
+    d1 = btot < L1_DEFAULT_BITS ? btot : L1_DEFAULT_BITS;
+  }
+  const uint64_t B1 = 1ull << d1;
+  // reads per block: up to 8 chunks of L1_NT, fewer for small inputs so that
+  // the grid still spreads over the CUs
+  const int rpb = L1_NT * (tun.l1_chunks > 0 && tun.l1_chunks <= 64
+                               ? tun.l1_chunks
+                               : (int)std::min<uint64_t>(8, std::max<uint64_t>(1, R / (L1_NT * 1024ull))));
+  const uint64_t nblk = (R + rpb - 1) / rpb;
+  // LDS staging: room for one record per thread of the input's mean size,
+  // + 25 %, for each of the two level-1 kernels
+  auto staging = [&](int nt, int cap) {
+    const double want = (double)nt * ((double)n / (double)R) * 1.25;
+    const int b = (int)std::min<double>(cap, std::max<double>(L1_SB_MIN, want));
+    return (b + 1023) & ~1023;
+  };
+  const int sb = staging(L1_NT, L1_SB_MAX);
+  constexpr int snt = L1Scatter<Key, ENC, OWN>::NT;
+  const int sbs = staging(snt, L1Scatter<Key, ENC, OWN>::SB_MAX);
+  ull *boff, *O;
+  uint32_t* H;
+  Key* S2;
+  PK_TRY(need(c, B_BOFF, B1 + 1, &boff));
+  PK_TRY(need(c, B_HIST1, B1 * nblk, &H));
+  PK_TRY(need(c, B_OFF1, B1 * nblk + 1, &O));
+  PK_TRY(need(c, B_S, N, &S));
+  PK_TRY(need(c, B_S2, N, &S2));
+  PK_TRY(need(c, B_C, N, &C));
+  c->timing.bucket_bits = btot;
+  PK_HIP(c, hipEventRecord(c->ev[2], st));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_hist<Key, ENC, OWN>), dim3((unsigned)nblk), dim3(L1_NT),
+                     l1_lds_bytes(sb, (int)B1, 4), st, fa, n, rstart, rlen, R, K, KB, mask, d1, rpb, sb, cf, H);
+  PK_HIP(c, hipGetLastError());
+  PK_HIP(c, hipEventRecord(c->ev[3], st));
+  PK_TRY(scan_excl<uint32_t>(c, H, O, B1 * nblk));
+  if (OWN) {
+    // this rank's instances
+    ull no = 0;
+    PK_TRY(d2h(c, &no, O + B1 * nblk, 8));
+    N = no;
+    c->N = N;
+    c->timing.n_instances = N;
+    if (N == 0) {
+      c->U = 0;
+      PK_TRY(need(c, B_TK, 1, &TK));
+      PK_TRY(need(c, B_TC, 1, &TC));
+      c->have_table = true;
+      return PK_OK;
+    }
+  }
+  hipLaunchKernelGGL(k_l1_boff, dim3(grid_for(B1 + 1, 256)), dim3(256), 0, st, O, (int)B1, nblk, (ull)N,
+                     boff);
+  PK_HIP(c, hipGetLastError());
+  PK_HIP(c, hipEventRecord(c->ev[4], st));
+  constexpr int sgf = L1Scatter<Key, ENC, OWN>::GF;
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key, ENC, OWN>), dim3((unsigned)((nblk + sgf - 1) / sgf)),
+                     dim3(snt), l1_lds_bytes(sbs, (int)B1, 12), st, fa, n, rstart, rlen, R, K, KB, mask, d1, rpb,
+                     sbs, cf, O, nblk, S);
+  PK_HIP(c, hipGetLastError());
+  PK_HIP(c, hipEventRecord(c->ev[16], st));
+  return count_partitioned<Key>(c, S, S2, C, boff, B1, d1, btot, KB, N);
+}
+
+// Distinct klen-byte windows of the implicit builder counters: keys from
+// k_win_keys (KB bits, even), then the partition and count from memory.
+template <class Key>
+int windows_count_impl(pk_ctx* c, const uint8_t* fa, const ull* rstart, const uint32_t* rlen, const ull* woff,
+                       uint64_t R, uint64_t N, int klen, int bits, const uint8_t* lut, int has_nul, int KB) {
+  hipStream_t st = c->stream;
+  c->R = R;
+  c->N = N;
+  c->k = klen;
+  c->words = sizeof(Key) / 8;
+  c->key_bits = -1;  // byte-rank codes: no macro nodes on this table
+  c->timing.n_instances = N;
+  c->timing.n_reads = R;
+  c->ms_levels = 0;
+  PK_HIP(c, hipEventRecord(c->ev[1], st));
+  if (N == 0) {
+    Key* TK;
+    int32_t* TC;
+    c->U = 0;
+    PK_TRY(need(c, B_TK, 1, &TK));
+    PK_TRY(need(c, B_TC, 1, &TC));
+    c->have_table = true;
+    return PK_OK;
+  }
+  Key *S, *S2;
+  int32_t* C;
+  ull *boff, *small;
+  PK_TRY(need(c, B_S, N, &S));
+  PK_TRY(need(c, B_S2, N, &S2));
+  PK_TRY(need(c, B_C, N, &C));
+  PK_TRY(need(c, B_BOFF, 2, &boff));
+  PK_TRY(need(c, B_SMALL, 16, &small));
+  PK_TRY(fill_u64(c, small + 4, 0, 1));
+  PK_TRY(fill_u64(c, boff, 0, 1));
+  PK_TRY(fill_u64(c, boff + 1, N, 1));
+  PK_HIP(c, hipEventRecord(c->ev[2], st));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_win_keys<Key>), dim3(grid_for(R, 256)), dim3(256), 0, st, fa, rstart, rlen,
+                     woff, R, klen, bits, lut, has_nul, S);
+  PK_HIP(c, hipGetLastError());
+  PK_HIP(c, hipEventRecord(c->ev[3], st));
+  PK_HIP(c, hipEventRecord(c->ev[4], st));
+  PK_HIP(c, hipEventRecord(c->ev[16], st));
+  const float bf = c->tuning.bucket_factor > 0.f ? c->tuning.bucket_factor : 3.0f;
+  const int btot = choose_bits(N, KB / 2, HashCfg<Key>::LIMIT, bf);
+  c->timing.bucket_bits = btot;
+  return count_partitioned<Key>(c, S, S2, C, boff, 1, 0, btot, KB, N);
+}
+
 template <class Key>
 int count_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K) {
   c->have_table = c->have_kept = c->have_nodes = false;
@@ -3074,7 +3276,24 @@ int rank_share_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K, int P, int 
   uint64_t R, N;
   PK_TRY(records_main(c, fa, n, K, &rstart, &rlen, &R, &N));
   return count_records<Key, ENC_MAIN, true>(c, fa, n, rstart, rlen, R, N, K, 2 * K, key_mask<Key>(2 * K),
-                                            OwnCfg{freq, P, rank});
+                                            OwnCfg{freq, P, rank, 0});
+}
+
+// One rank's share when every k-mer belongs to rank retrieve_proc_id(kmer)
+// (the -p counters, kmer_counting.cpp:1825-2091, 2095-2271).
+template <class Key>
+int rank_share_key_impl(pk_ctx* c, const uint8_t* fa, uint64_t n, int K, int P, int rank) {
+  c->have_table = c->have_kept = c->have_nodes = false;
+  c->timing = pk_timing{};
+  c->pend_count = c->pend_split = c->pend_cutoff = c->pend_mn = c->pend_norm = false;
+  c->key_bits = 2 * K;
+  PK_HIP(c, hipEventRecord(c->ev[0], c->stream));
+  ull* rstart;
+  uint32_t* rlen;
+  uint64_t R, N;
+  PK_TRY(records_main(c, fa, n, K, &rstart, &rlen, &R, &N));
+  return count_records<Key, ENC_MAIN, true>(c, fa, n, rstart, rlen, R, N, K, 2 * K, key_mask<Key>(2 * K),
+                                            OwnCfg{nullptr, P, rank, 1});
 }
 
 template <class Key>
@@ -3223,6 +3442,35 @@ int mn_impl(pk_ctx* c) {
   c->pend_mn = true;
   c->M = M;
   c->have_nodes = true;
+  return PK_OK;
+}
+
+// Keeps the rank's share of the node table (in place, key order kept).
+template <class Key>
+int node_share_impl(pk_ctx* c, int P, int rank) {
+  hipStream_t st = c->stream;
+  const uint64_t M = c->M;
+  MnRec* nodes = reinterpret_cast<MnRec*>(c->buf[B_NODES].p);
+  uint32_t* flag;
+  ull* pos;
+  MnRec* tmp;
+  PK_TRY(need(c, B_TU, M + 1, &flag));
+  PK_TRY(need(c, B_UOFF, M + 2, &pos));
+  PK_TRY(need(c, B_MNNT, M + 1, &tmp));
+  if (M) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_node_share_flags<Key>), dim3(grid_for(M, 256)), dim3(256), 0, st, nodes,
+                       M, P, rank, flag);
+    PK_HIP(c, hipGetLastError());
+  }
+  PK_TRY(scan_excl<uint32_t>(c, flag, pos, M));
+  ull M2 = 0;
+  PK_TRY(d2h(c, &M2, pos + M, 8));
+  if (M) {
+    hipLaunchKernelGGL(k_node_share_write, dim3(grid_for(M, 256)), dim3(256), 0, st, nodes, M, flag, pos, tmp);
+    PK_HIP(c, hipGetLastError());
+  }
+  if (M2) PK_HIP(c, hipMemcpyAsync(nodes, tmp, M2 * sizeof(MnRec), hipMemcpyDeviceToDevice, st));
+  c->M = M2;
   return PK_OK;
 }
 
@@ -3569,6 +3817,28 @@ int pk_count_rank_share(pk_ctx* c, const char* fasta, uint64_t nbytes, int k, in
   return PK_OK;
 }
 
+int pk_count_rank_share_by_key(pk_ctx* c, const char* fasta, uint64_t nbytes, int k, int nranks, int rank,
+                               int fasta_on_device, pk_table* out) {
+  if (!c || !out || (!fasta && nbytes)) return PK_EINVAL;
+  if (k < 8 || k > 64) return c->fail(PK_EINVAL, "k must be in [8, 64]");
+  if (nranks < 1 || rank < 0 || rank >= nranks) return c->fail(PK_EINVAL, "need 0 <= rank < nranks");
+  c->timing = pk_timing{};
+  c->pend_count = c->pend_split = false;
+  if (nranks == 1) return pk_count(c, fasta, nbytes, k, fasta_on_device, out);
+  PK_HIP(c, hipSetDevice(c->device));
+  const uint8_t* fa;
+  PK_TRY(fasta_to_device(c, fasta, nbytes, fasta_on_device, &fa));
+  int rc = (k <= 32) ? rank_share_key_impl<uint64_t>(c, fa, nbytes, k, nranks, rank)
+                     : rank_share_key_impl<u128>(c, fa, nbytes, k, nranks, rank);
+  if (rc) return rc;
+  out->n = c->U;
+  out->words = c->words;
+  out->k = c->k;
+  out->keys = reinterpret_cast<const uint64_t*>(c->buf[B_TK].p);
+  out->counts = reinterpret_cast<const int32_t*>(c->buf[B_TC].p);
+  return PK_OK;
+}
+
 int pk_lmer_frequency(pk_ctx* c, const char* fasta, uint64_t nbytes, int fasta_on_device,
                       uint64_t* freq_host) {
   if (!c || !freq_host || (!fasta && nbytes)) return PK_EINVAL;
@@ -3603,6 +3873,103 @@ int pk_builder_count(pk_ctx* c, const char* fasta, uint64_t nbytes, int klen, ui
   out->keys = reinterpret_cast<const uint64_t*>(c->buf[B_TK].p);
   out->counts = reinterpret_cast<const int32_t*>(c->buf[B_TC].p);
   if (n_reads) *n_reads = c->R;
+  return PK_OK;
+}
+
+int pk_builder_count_windows(pk_ctx* c, const char* fasta, uint64_t nbytes, int klen, uint64_t read_len,
+                             char delimiter, char skip_symbol, int mode, int fasta_on_device, uint64_t* n_reads,
+                             uint64_t* n_windows, uint64_t* n_distinct) {
+  if (!c || !n_reads || !n_windows || !n_distinct || (!fasta && nbytes)) return PK_EINVAL;
+  if (mode != PK_WINDOWS_V0 && mode != PK_WINDOWS_V1) return c->fail(PK_EINVAL, "unknown window mode");
+  if (klen < 1 || klen > 128) return c->fail(PK_EINVAL, "klen must be in [1, 128]");
+  if (read_len > 0xFFFFFFF0ull) return c->fail(PK_EINVAL, "read_len must be < 2^32 - 16");
+  if (read_len < (uint64_t)klen)
+    return c->fail(PK_EINVAL, "read_len < klen: the reference's window count wraps below zero");
+  if (delimiter == skip_symbol) return c->fail(PK_EINVAL, "delimiter and skip_symbol must differ");
+  c->timing = pk_timing{};
+  c->pend_count = c->pend_split = c->pend_cutoff = c->pend_mn = c->pend_norm = false;
+  c->have_table = c->have_kept = c->have_nodes = false;
+  const BldSyms sy{(uint8_t)delimiter, (uint8_t)skip_symbol};
+  PK_HIP(c, hipSetDevice(c->device));
+  const uint8_t* fa;
+  PK_TRY(fasta_to_device(c, fasta, nbytes, fasta_on_device, &fa));
+  hipStream_t st = c->stream;
+  PK_HIP(c, hipEventRecord(c->ev[0], st));
+  ull* small;
+  PK_TRY(need(c, B_SMALL, 16, &small));
+  PK_TRY(fill_u64(c, small, ~0ull, 4));
+  PK_TRY(fill_u64(c, small + 12, 0, 2));
+  ull* ev;
+  uint64_t m, p0;
+  PK_TRY(event_index(c, fa, nbytes, sy.delim, sy.skip, small, &ev, &m, &p0));
+  ull *rstart, *woff, *idx;
+  uint32_t *rlen, *wc, *flag;
+  uint32_t* bad = reinterpret_cast<uint32_t*>(small + 12);
+  uint32_t* present = reinterpret_cast<uint32_t*>(small + 8);  // 8 words = small[8..11]
+  PK_TRY(need(c, B_TU, m + 1, &flag));
+  PK_TRY(need(c, B_UOFF, m + 2, &idx));
+  if (m) {
+    hipLaunchKernelGGL(k_win_flags, dim3(grid_for(m, 256)), dim3(256), 0, st, fa, nbytes, ev, m, klen, read_len, sy,
+                       flag, bad);
+    PK_HIP(c, hipGetLastError());
+  }
+  PK_TRY(scan_excl<uint32_t>(c, flag, idx, m));
+  ull hv[2];
+  PK_HIP(c, hipMemcpyAsync(&hv[0], idx + m, 8, hipMemcpyDeviceToHost, st));
+  PK_HIP(c, hipMemcpyAsync(&hv[1], bad, 8, hipMemcpyDeviceToHost, st));
+  PK_HIP(c, hipStreamSynchronize(st));
+  const uint64_t R = hv[0];
+  const bool cut = (hv[1] & 0xFFFFFFFFull) != 0, shorter = (hv[1] >> 32) != 0;
+  // concurrent_kmer_counter.hpp:965-967 throws on a read cut short; _v1
+  // sizes its window array with read_len - klen + 1 (:1427), which wraps
+  // for a read shorter than klen
+  if (mode == PK_WINDOWS_V0 && cut) return c->fail(PK_EFORMAT, "Variable size read");
+  if (mode == PK_WINDOWS_V1 && shorter) return c->fail(PK_EFORMAT, "a read is shorter than klen");
+  PK_TRY(need(c, B_RSTART, R + 1, &rstart));
+  PK_TRY(need(c, B_RLEN, R + 1, &rlen));
+  PK_TRY(need(c, B_KC, R + 1, &wc));
+  PK_TRY(need(c, B_KOFS, R + 2, &woff));
+  if (m) {
+    hipLaunchKernelGGL(k_win_write, dim3(grid_for(m, 256)), dim3(256), 0, st, fa, nbytes, ev, m, klen, read_len, sy,
+                       idx, rstart, rlen, wc);
+    PK_HIP(c, hipGetLastError());
+  }
+  PK_TRY(scan_excl<uint32_t>(c, wc, woff, R));
+  PK_TRY(fill_u64(c, small + 8, 0, 4));
+  if (R) {
+    hipLaunchKernelGGL(k_win_alpha, dim3((unsigned)std::min<uint64_t>((R + 3) / 4, 4096)), dim3(64, 4), 0, st, fa,
+                       rstart, rlen, R, present);
+    PK_HIP(c, hipGetLastError());
+  }
+  ull hp[5];
+  PK_HIP(c, hipMemcpyAsync(hp, small + 8, 32, hipMemcpyDeviceToHost, st));
+  PK_HIP(c, hipMemcpyAsync(&hp[4], woff + R, 8, hipMemcpyDeviceToHost, st));
+  PK_HIP(c, hipStreamSynchronize(st));
+  const uint64_t N = hp[4];
+  // byte value -> rank among the values present
+  const uint32_t* pw = reinterpret_cast<const uint32_t*>(hp);
+  int nval = 0;
+  for (int v = 0; v < 256; v++) {
+    c->win_lut[v] = 0;
+    if (pw[v >> 5] >> (v & 31) & 1u) c->win_lut[v] = (uint8_t)nval++;
+  }
+  int bits = 1;
+  while ((1 << bits) < nval) bits++;
+  int KB = klen * bits;
+  if (KB > 128)
+    return c->fail(PK_EINVAL, "the reads hold " + std::to_string(nval) + " byte values: " + std::to_string(klen) +
+                                  " x " + std::to_string(bits) + " bits exceed a 128-bit window key");
+  KB += KB & 1;
+  const int has_nul = (pw[0] & 1u) ? 1 : 0;
+  uint8_t* lut;
+  PK_TRY(need(c, B_WORDER, 256, &lut));
+  PK_HIP(c, hipMemcpyAsync(lut, c->win_lut, 256, hipMemcpyHostToDevice, st));
+  int rc = KB <= 64 ? windows_count_impl<uint64_t>(c, fa, rstart, rlen, woff, R, N, klen, bits, lut, has_nul, KB)
+                    : windows_count_impl<u128>(c, fa, rstart, rlen, woff, R, N, klen, bits, lut, has_nul, KB);
+  if (rc) return rc;
+  *n_reads = R;
+  *n_windows = N;
+  *n_distinct = c->U;
   return PK_OK;
 }
 
@@ -3666,6 +4033,17 @@ int pk_macro_nodes(pk_ctx* c, pk_mn_table* out) {
   out->n = c->M;
   out->k = c->k;
   out->nodes = reinterpret_cast<const pk_mnode*>(c->buf[B_NODES].p);
+  return PK_OK;
+}
+
+int pk_macro_nodes_rank_share(pk_ctx* c, int nranks, int rank, pk_mn_table* out) {
+  if (!c || !out) return PK_EINVAL;
+  if (nranks < 1 || rank < 0 || rank >= nranks) return c->fail(PK_EINVAL, "need 0 <= rank < nranks");
+  PK_TRY(pk_macro_nodes(c, out));
+  if (nranks == 1) return PK_OK;
+  int rc = c->words == 1 ? node_share_impl<uint64_t>(c, nranks, rank) : node_share_impl<u128>(c, nranks, rank);
+  if (rc) return rc;
+  out->n = c->M;
   return PK_OK;
 }
 

@@ -13,6 +13,13 @@
 //                                              (the reference unit test's call,
 //                                              src/concurrent_kmer_counter_test.cpp:94)
 //
+//   builder_adapter_test explicit|implicit_v0|implicit_v1 FILE KLEN READ_LEN BITS
+//                                              the counters src/concurrent_kmer_table_builder.cpp:
+//                                              97-116 calls (processor never called):
+//                                              their own "unique kmers=" and
+//                                              "concurrent_vector size=" lines,
+//                                              then "reads=R calls=C"
+//
 // Prints one line "reads=R unique=U freq=F calls=C xor=X" (X = xor of
 // key words and counts over all pairs seen, an order-independent digest).
 #include <atomic>
@@ -71,6 +78,19 @@ int run_v3(const std::string& input, size_t klen, size_t read_len, size_t max_km
   return 0;
 }
 
+template <class kmer_t>
+int run_table_builder_mode(const std::string& mode, const std::string& input, size_t klen, size_t read_len) {
+  pakhip::Hip_Kmer_Partial_Counter<kmer_t> counter(klen, input.data(), input.data() + input.size(), read_len,
+                                                   std::numeric_limits<size_t>::max());
+  count_unique_kmers_t p;
+  size_t reads;
+  if (mode == "explicit") reads = counter.concurrent_count_and_process_omp_explicit(p);
+  else if (mode == "implicit_v0") reads = counter.concurrent_count_and_process_omp_implicit_v0(p);
+  else reads = counter.concurrent_count_and_process_omp_implicit_v1(p);
+  printf("reads=%zu calls=%zu\n", reads, (size_t)p.calls_);
+  return 0;
+}
+
 static std::string read_file(const char* path) {
   FILE* f = fopen(path, "rb");
   if (!f) throw std::logic_error(std::string("Unable to open file: ") + path);
@@ -119,6 +139,13 @@ int main(int argc, char** argv) {
         "TACCATTCACCCGCTGGCGGGGCAGGGGGTAAATCTCGGCTTTATGGATGCTGCAGAGCTGATTGCCGAACTGAAACGGTTGCATCGTCAGGGGAAAGAC\n"
         ">CACVBX020000001.1-NC_000913.3519\n"
         "TGGTTTGGCCGTCGCTTAGGCCAGGATGCCGCACCAGAGAAACTGGGCGTTGACCGCGCACTGGCATTCAGAGAAATCAATCAGCAACTTTATCAACTAC\n";
+    if (argc == 6 && (std::string(argv[1]) == "explicit" || std::string(argv[1]) == "implicit_v0" ||
+                      std::string(argv[1]) == "implicit_v1")) {
+      const std::string input = read_file(argv[2]);
+      const size_t klen = strtoull(argv[3], nullptr, 10), read_len = strtoull(argv[4], nullptr, 10);
+      return atoi(argv[5]) == 64 ? run_table_builder_mode<size_t>(argv[1], input, klen, read_len)
+                                 : run_table_builder_mode<unsigned __int128>(argv[1], input, klen, read_len);
+    }
     if (argc >= 2 && std::string(argv[1]) == "v3") {
       if (argc == 2) return run_v3<size_t>(unit_input, 32, 100, std::numeric_limits<size_t>::max());
       if (argc != 7) {

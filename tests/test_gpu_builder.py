@@ -12,7 +12,7 @@ import sys
 import numpy as np
 import pytest
 
-from builder_util import edge_inputs
+from builder_util import builder_closed_form, edge_inputs, windows_closed_form
 from golden_util import read_fasta
 
 pytestmark = pytest.mark.gpu
@@ -207,3 +207,86 @@ def test_cutoff_threshold(ctx, oracle_mod):
         kk, kc = ctx.kept_host()
         sel = counts > thr  # pakman_graph.hpp:603 skips count <= threshold
         assert nk == int(sel.sum()) and np.array_equal(kk, keys[sel]) and np.array_equal(kc, counts[sel])
+
+
+# -- the counters concurrent_kmer_table_builder.cpp:97-116 calls ---------------
+
+WINDOW_PARAMS = [(4, 10), (11, 30), (21, 100), (2, 160)]
+
+
+@pytest.mark.parametrize("name", sorted(edge_inputs()))
+@pytest.mark.parametrize("klen,read_len", WINDOW_PARAMS)
+@pytest.mark.parametrize("mode", ["v0", "v1"])
+def test_builder_windows_edge_inputs(ctx, name, klen, read_len, mode):
+    """pk_builder_count_windows against the strncmp-window restatement."""
+    from pakman_plus_amd import api
+    _p(f"case: builder windows {name} klen={klen} read_len={read_len} {mode}")
+    fa = edge_inputs()[name]
+    try:
+        want = windows_closed_form(fa, klen, read_len, mode)
+    except ValueError:
+        with pytest.raises(api.PakHipError) as e:
+            ctx.builder_count_windows(fa, klen, read_len, mode)
+        assert e.value.code == api.PK_EFORMAT
+        return
+    try:
+        got = ctx.builder_count_windows(fa, klen, read_len, mode)
+    except api.PakHipError as e:
+        # more byte values than a 128-bit key holds at this klen
+        assert e.code == api.PK_EINVAL and "128-bit" in str(e)
+        assert len(set(fa)) > 2 ** (128 // klen)
+        return
+    assert got == want
+
+
+@pytest.mark.parametrize("klen,read_len", [(32, 100), (32, 150), (64, 150), (21, 100)])
+def test_builder_windows_read_sets(ctx, klen, read_len):
+    _p(f"case: builder windows syn_mid klen={klen} read_len={read_len}")
+    fa = read_fasta("syn_mid.fa.gz")
+    for mode in ("v0", "v1"):
+        assert ctx.builder_count_windows(fa, klen, read_len, mode) == windows_closed_form(fa, klen, read_len, mode)
+
+
+def test_builder_windows_nul_bytes(ctx):
+    _p("case: builder windows with NUL bytes")
+    fa = b">a\nAC\x00GTACGT\n>b\nAC\x00TTACGT\n>c\n\x00\x00ACGTACG\n"
+    for klen in (2, 4, 7):
+        assert ctx.builder_count_windows(fa, klen, 9, "v0") == windows_closed_form(fa, klen, 9, "v0")
+
+
+def _run_mode(mode, path, klen, read_len, bits):
+    out = subprocess.run([ADAPTER, mode, str(path), str(klen), str(read_len), str(bits)], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    r = {}
+    for ln in out.stdout.splitlines():
+        if ln.startswith("unique kmers="):
+            r["unique"] = int(ln.split("=")[1])
+        elif ln.startswith("concurrent_vector size="):
+            r["size"] = int(ln.split("=")[1])
+        elif ln.startswith("reads="):
+            r.update({k: int(v) for k, v in (f.split("=") for f in ln.split())})
+    return r
+
+
+@pytest.mark.parametrize("klen,bits", [(32, 64), (64, 128)])
+def test_cpp_adapter_table_builder_modes(tmp_path, klen, bits):
+    """The three counters of concurrent_kmer_table_builder.cpp:97-116 through
+    the C++ adapter: reads returned, processor not called, the reference's
+    "unique kmers" / "concurrent_vector size" figures at one thread."""
+    _p(f"case: C++ adapter table-builder modes klen={klen}")
+    fa = read_fasta("syn_mid.fa.gz")
+    f = tmp_path / "mid.fa"
+    f.write_bytes(fa)
+    reads, counts = builder_closed_form(fa, klen, 100, bits)
+    r = _run_mode("explicit", f, klen, 100, bits)
+    assert r == {"unique": len(counts), "size": sum(counts.values()), "reads": reads, "calls": 0}
+    for mode in ("v0", "v1"):
+        wr, ww, wd = windows_closed_form(fa, klen, 100, mode)
+        r = _run_mode("implicit_" + mode, f, klen, 100, bits)
+        assert r == {"unique": wd, "size": ww, "reads": wr, "calls": 0}
+    # an empty range: no k-mer, the reference's one-thread figure is 1
+    e = tmp_path / "empty.fa"
+    e.write_bytes(b"")
+    assert _run_mode("explicit", e, klen, 100, bits) == {"unique": 1, "size": 0, "reads": 0, "calls": 0}
+    assert _run_mode("implicit_v1", e, klen, 100, bits) == {"unique": 1, "size": 0, "reads": 0, "calls": 0}

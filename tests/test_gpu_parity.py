@@ -312,6 +312,74 @@ def test_rank_share(ctx, oracle_mod, k, P):
             assert total >= ctx.table.n
 
 
+def test_rank_share_non_acgt_vs_restatement(ctx):
+    """Min-L-mer routing and L-mer frequencies on reads with non-ACGT bytes,
+    against the window-by-window restatement of tests/rank_util.py (not the
+    oracle)."""
+    import rank_util as RU
+    from test_rank_cpu import NON_ACGT as FA
+    _p("case: rank share non-ACGT vs restatement")
+    freq = RU.lmer_freq(FA)
+    assert np.array_equal(ctx.lmer_frequency(FA), freq)
+    for k, P in ((8, 3), (21, 3), (32, 2), (33, 5), (64, 3)):
+        walk = RU.owner_walk(FA, k, P, freq)
+        for r in range(P):
+            ctx.count_rank_share(FA, k, P, r)
+            keys, counts = ctx.table_host()
+            assert RU.table_counter(keys, counts) == RU.share_counter(walk, r), (k, P, r)
+
+
+@pytest.mark.parametrize("k,P", [(21, 3), (32, 2), (32, 8), (33, 3), (64, 5)])
+def test_rank_share_by_key(ctx, oracle_mod, k, P):
+    """The -p counters' owner rule (retrieve_proc_id of the whole k-mer,
+    kmer_counting.cpp:1949-1950): each share is the one-rank table filtered
+    by the closed-form owner (hash31 / uhash31 of tests/rank_util.py), and
+    the shares partition the table."""
+    import rank_util as RU
+    _p(f"case: rank share by key k={k} P={P}")
+    for fa in (read_fasta("syn_mid.fa.gz"), NON_ACGT):
+        ok, oc = oracle_mod.count(fa, k)
+        full = RU.table_counter(ok, oc)
+        total = 0
+        for r in range(P):
+            ctx.count_rank_share_by_key(fa, k, P, r)
+            keys, counts = ctx.table_host()
+            got = RU.table_counter(keys, counts)
+            assert got == {x: c for x, c in full.items() if RU.key_owner(x, k, P) == r}, r
+            # ascending within the share
+            vals = [int(x[0]) | ((int(x[1]) << 64) if keys.shape[1] == 2 else 0) for x in keys]
+            assert vals == sorted(set(vals))
+            total += len(got)
+        assert total == len(full)
+
+
+@pytest.mark.parametrize("name,k,t,P", [("syn_mid.fa.gz", 32, 21, 3), ("syn_rep.fa.gz", 64, 15, 4),
+                                        ("syn_tiny.fa.gz", 33, 21, 2)])
+def test_macro_nodes_rank_share(ctx, oracle_mod, name, k, t, P):
+    """The nodes a rank holds at P ranks: node keys owned by retrieve_proc_id
+    (hash31 for k <= 32, uhash31 above), each with its one-rank record."""
+    import rank_util as RU
+    _p(f"case: macro-node rank share {name} k={k} P={P}")
+    from pakman_plus_amd import api
+    fa = read_fasta(name)
+    ok, oc = oracle_mod.count(fa, k)
+    _, _, keep, _ = oracle_mod.cutoff(oc, t)
+    want = oracle_mod.macro_nodes(ok[keep], oc[keep], k)
+    wkeys = [int(x[0]) | ((int(x[1]) << 64) if k > 32 else 0) for x in want["keys"]]
+    seen = 0
+    for r in range(P):
+        ctx.count(fa, k)
+        ctx.cutoff(t)
+        m = ctx.macro_nodes_rank_share(P, r)
+        got = api.nodes_as_arrays(ctx.nodes_host(), k)
+        sel = np.array([RU.key_owner(x, k, P) == r for x in wkeys], dtype=bool)
+        assert m == int(sel.sum())
+        for f in ("keys", "pre_cnt", "suf_cnt"):
+            assert np.array_equal(got[f], want[f][sel]), (r, f)
+        seen += m
+    assert seen == len(wkeys)
+
+
 @pytest.mark.parametrize("tuning", [dict(l1_chunks=1, l1_bits=8, bucket_factor=1.5),
                                     dict(l1_chunks=3, l1_bits=6, bucket_factor=5.0),
                                     dict(l1_chunks=8, l1_bits=4, bucket_factor=0.5),
