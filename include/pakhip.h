@@ -137,6 +137,16 @@ typedef struct pk_tuning {
 } pk_tuning;
 int pk_ctx_set_tuning(pk_ctx* ctx, const pk_tuning* t);
 
+/* How the context allocates its device buffers.  PK_ALLOC_DEVICE (the
+ * default): hipMalloc / hipFree, where a buffer that grows waits for the
+ * whole device.  PK_ALLOC_STREAM_ORDERED: hipMallocAsync / hipFreeAsync on the
+ * context's stream from the device's default pool, so a context that grows
+ * a buffer does not wait on other contexts' work (the server's slots use
+ * it).  Changing the mode frees the context's buffers. */
+#define PK_ALLOC_DEVICE 0
+#define PK_ALLOC_STREAM_ORDERED 1
+int pk_ctx_set_alloc_mode(pk_ctx* ctx, int mode);
+
 /* Free every device buffer of the context (they grow back on the next call);
  * invalidates the table, kept table and node pointers. */
 int pk_ctx_release_buffers(pk_ctx* ctx);
@@ -147,6 +157,13 @@ int pk_ctx_release_buffers(pk_ctx* ctx);
  * resident.  8 <= k <= 64.  On success *out describes the table. */
 int pk_count(pk_ctx* ctx, const char* fasta, uint64_t nbytes, int k, int fasta_on_device,
              pk_table* out);
+
+/* pk_count of a FASTA file (the reference's input path: its rank's byte
+ * range read with MPI-IO, DivideReads, src/input_reading.cpp:69-243; one GPU
+ * takes the whole file).  Host threads read slices of the file into pinned
+ * buffers while the previous chunks are copied to HBM.  A file that cannot be
+ * opened or read fails with PK_EINVAL. */
+int pk_count_file(pk_ctx* ctx, const char* path, int k, pk_table* out);
 
 /* Allocation-free count.  No counterpart in the reference: for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving

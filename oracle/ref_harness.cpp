@@ -23,7 +23,8 @@
 // the MN_map is filled from those records by the library's filler
 // (include/pakhip_mnmap.hpp, pakhip::fill_MN_map) and the reference's
 // wiring, compaction and walk run on it.  The contigs then show whether the
-// records plus the filler are a drop-in for construct_macro_nodes.
+// records plus the filler are a drop-in for construct_macro_nodes.  The
+// fill's time is printed; $PAKREF_FILL_ONLY stops after the fill.
 
 #include <array>
 #include <cstdio>
@@ -130,8 +131,13 @@ static bool fill_from_records(const char* path, std::vector<std::pair<kmer_t, Ma
   if (ok && n) ok = fread(recs.data(), sizeof(pk_mnode), n, f) == n && fread(order.data(), 4, n, f) == n;
   fclose(f);
   if (!ok) { fprintf(stderr, "%s: not a node-record file for k=%d\n", path, KMER_LENGTH); return false; }
+  const double t0 = MPI_Wtime();
   pakhip::fill_MN_map<kmer_t, MacroNode>(recs.data(), recs.size(), k, coverage, order.data(), MN_map);
-  if (rank == 0) printf("Total number of macro_nodes across all proc's: %lu\n", (unsigned long)MN_map.size());
+  const double t1 = MPI_Wtime();
+  if (rank == 0) {
+    printf("Total number of macro_nodes across all proc's: %lu\n", (unsigned long)MN_map.size());
+    printf("MN_map fill from node records (secs): %f\n", t1 - t0);
+  }
   return true;
 }
 
@@ -148,6 +154,10 @@ int main(int argc, char** argv) {
   const char* recs = getenv("PAKREF_NODES");
   if (recs) {
     if (!fill_from_records(recs, MN_map)) return 2;
+    if (getenv("PAKREF_FILL_ONLY")) {
+      MPI_Finalize();
+      return 0;
+    }
   } else {
   FILE* fp = fopen(inputFileName.c_str(), "rb");
   if (!fp) { perror(inputFileName.c_str()); return 2; }

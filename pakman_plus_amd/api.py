@@ -86,8 +86,9 @@ MNODE_DTYPE = np.dtype([("key_lo", "<u8"), ("key_hi", "<u8"), ("pre_cnt", "<i4",
 
 # every symbol declared in include/pakhip.h
 EXPORTS = ("pk_ctx_create", "pk_ctx_destroy", "pk_last_error", "pk_ctx_stream",
-           "pk_ctx_device_bytes", "pk_ctx_set_tuning", "pk_ctx_release_buffers", "pk_count",
-           "pk_count_noalloc",
+           "pk_ctx_device_bytes", "pk_ctx_set_tuning", "pk_ctx_release_buffers", "pk_ctx_set_alloc_mode",
+           "pk_count",
+           "pk_count_noalloc", "pk_count_file",
            "pk_count_format", "pk_normalize", "pk_download_normalized",
            "pk_count_rank_share", "pk_count_rank_share_by_key", "pk_macro_nodes_rank_share",
            "pk_lmer_frequency", "pk_builder_count", "pk_builder_count_windows",
@@ -131,6 +132,8 @@ def load_library():
     L.pk_ctx_device_bytes.restype = u64
     L.pk_count.argtypes = [vp, vp, u64, i32, i32, ctypes.POINTER(_Table)]
     L.pk_count.restype = i32
+    L.pk_count_file.argtypes = [vp, ctypes.c_char_p, i32, ctypes.POINTER(_Table)]
+    L.pk_count_file.restype = i32
     L.pk_count_noalloc.argtypes = [vp, vp, u64, i32, vp, vp, u64, ctypes.POINTER(u64)]
     L.pk_count_noalloc.restype = i32
     L.pk_count_format.argtypes = [vp, vp, u64, i32, i32, i32, ctypes.POINTER(_Table)]
@@ -175,6 +178,8 @@ def load_library():
     L.pk_ctx_set_tuning.restype = i32
     L.pk_server_set_tuning.argtypes = [vp, ctypes.POINTER(_TuneEntry), i32]
     L.pk_server_set_tuning.restype = i32
+    L.pk_ctx_set_alloc_mode.argtypes = [vp, i32]
+    L.pk_ctx_set_alloc_mode.restype = i32
     L.pk_ctx_release_buffers.argtypes = [vp]
     L.pk_ctx_release_buffers.restype = i32
     L.pk_server_create.argtypes = [ctypes.POINTER(vp), i32, i32, u64]
@@ -261,6 +266,11 @@ class Context:
     def device_bytes(self) -> int:
         return int(self._lib.pk_ctx_device_bytes(self._h))
 
+    def set_alloc_mode(self, stream_ordered: bool):
+        """Buffers from the device's stream-ordered pool (True) or hipMalloc
+        (False, the default); changing the mode frees the buffers."""
+        self._check(self._lib.pk_ctx_set_alloc_mode(self._h, 1 if stream_ordered else 0))
+
     def release_buffers(self):
         """Free the context's device buffers (they grow back on the next call)."""
         self._check(self._lib.pk_ctx_release_buffers(self._h))
@@ -283,6 +293,15 @@ class Context:
             a = _as_u8(fasta)
             rc = self._lib.pk_count(self._h, a.ctypes.data, a.nbytes, k, 0, ctypes.byref(t))
         self._check(rc)
+        self.k = k
+        self.table = TableView(int(t.n), int(t.words), int(t.k), t.keys or 0, t.counts or 0)
+        return self.table
+
+    def count_file(self, path: str, k: int) -> TableView:
+        """Count the k-mers of a FASTA file read straight into HBM through
+        pinned staging (pk_count_file)."""
+        t = _Table()
+        self._check(self._lib.pk_count_file(self._h, os.fsencode(path), k, ctypes.byref(t)))
         self.k = k
         self.table = TableView(int(t.n), int(t.words), int(t.k), t.keys or 0, t.counts or 0)
         return self.table

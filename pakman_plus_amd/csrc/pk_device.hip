@@ -38,7 +38,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <cerrno>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -717,6 +722,7 @@ __device__ __forceinline__ uint32_t base_code(uint32_t c) {
 
 constexpr int L1_NT = 512;  // level-1 histogram blocks
 constexpr int MS_NT = 512;
+constexpr int MS_P_NT = 512;  // threads of a persistent multisplit block (one tile at a time)
 constexpr int L1_MAX_BITS = PK_TUNING_L1_BITS_MAX;  // level-1 digit width (direct scatter)
 constexpr int L1_DEFAULT_BITS = 8;  // wider digits keep more partial lines open
 constexpr int MS_MAX_BITS = 8;   // further levels (tile split into runs)
@@ -1209,12 +1215,13 @@ __global__ __launch_bounds__(MS_NT) void k_ms_scatter(const Key* Sin, const ull*
 // of H in (bucket, digit, tile) order), so bases are read from O only for a
 // block's first tile in each bucket: one line per digit per bucket instead of
 // per tile.
-template <class Key>
-__global__ __launch_bounds__(MS_NT, 4) void k_ms_scatter_p(const Key* Sin, const ull* boff, const ull* tp,
+template <class Key, int NT>
+__global__ __launch_bounds__(NT, NT >= 1024 ? 8 : 4) void k_ms_scatter_p(const Key* Sin, const ull* boff, const ull* tp,
                                                            int nb, int dpos, int dbits, const ull* O,
                                                            Key* Sout) {
-  constexpr int Q = TileCfg<Key>::Q;
+  // NT threads share one tile of TQ keys (the level's tile), Q per thread
   constexpr int TQ = MS_NT * TileCfg<Key>::Q;
+  constexpr int Q = TQ / NT;
   __shared__ TileLds<Key, MS_NT> L;
   const ull ntl = tp[nb];
   const ull per = (ntl + gridDim.x - 1) / gridDim.x;
@@ -1223,7 +1230,7 @@ __global__ __launch_bounds__(MS_NT, 4) void k_ms_scatter_p(const Key* Sin, const
   if (g0 >= g1) return;
   const int ns = 1 << dbits;
   const int tid = (int)threadIdx.x;
-  // the current tile's keys (tid, tid + MS_NT, ...) and this thread's
+  // the current tile's keys (tid, tid + NT, ...) and this thread's
   // sub-digit base (tid < ns)
   int a = tile_owner(tp, nb, g0);
   ull t, b0;
@@ -1232,7 +1239,7 @@ __global__ __launch_bounds__(MS_NT, 4) void k_ms_scatter_p(const Key* Sin, const
   Key kr[Q];
 #pragma unroll
   for (int j = 0; j < Q; j++) {
-    const int idx = tid + j * MS_NT;
+    const int idx = tid + j * NT;
     kr[j] = idx < m ? Sin[b0 + idx] : Key(0);
   }
   ull bs = tid < ns ? O[ms_index(tp, a, tid, t, dbits)] : 0ull;
@@ -1244,14 +1251,14 @@ __global__ __launch_bounds__(MS_NT, 4) void k_ms_scatter_p(const Key* Sin, const
       ms_tile(boff, tp, g + 1, TQ, an, tn, b0n, mn);
 #pragma unroll
       for (int j = 0; j < Q; j++) {
-        const int idx = tid + j * MS_NT;
+        const int idx = tid + j * NT;
         kn[j] = idx < mn ? Sin[b0n + idx] : Key(0);
       }
       // a new bucket: its first base from the scan
       if (tid < ns && an != a) bn = O[ms_index(tp, an, tid, tn, dbits)];
     }
     if (tid < ns) L.h[tid] = 0;
-    const int np = m > tid ? (m - tid - 1) / MS_NT + 1 : 0;
+    const int np = m > tid ? (m - tid - 1) / NT + 1 : 0;
     __syncthreads();
     // ranks inside each digit from the histogram's atomics (see k_ms_scatter)
     uint32_t rk[(Q + 1) / 2] = {};
@@ -1262,7 +1269,7 @@ __global__ __launch_bounds__(MS_NT, 4) void k_ms_scatter_p(const Key* Sin, const
     const uint32_t v = tid < ns ? L.h[tid] : 0u;
     // the next tile of the same bucket starts its runs where this one's end
     if (an == a) bn = bs + v;
-    const ull ex = block_excl_scan<MS_NT>(v);
+    const ull ex = block_excl_scan<NT>(v);
     if (tid < ns) {
       L.tcur[tid] = (uint32_t)ex;
       L.delta[tid] = (long long)bs - (long long)ex;
@@ -1276,7 +1283,7 @@ __global__ __launch_bounds__(MS_NT, 4) void k_ms_scatter_p(const Key* Sin, const
       }
     }
     __syncthreads();
-    for (int p = tid; p < m; p += MS_NT) {
+    for (int p = tid; p < m; p += NT) {
       const Key v = L.T1[p];
       Sout[L.delta[key_digit<Key>(v, dpos, dbits)] + p] = v;
     }
@@ -2464,6 +2471,8 @@ struct pk_ctx {
   uint64_t arena_cap = 0, arena_used = 0;
   bool arena_fixed[B_COUNT] = {};
   uint8_t win_lut[256] = {};  // byte -> code of the last pk_builder_count_windows
+  // buffers from the device's stream-ordered pool (pk_ctx_set_alloc_mode)
+  bool stream_alloc = false;
 
   int fail(int code, const std::string& what) {
     err = what;
@@ -2514,7 +2523,8 @@ int need(pk_ctx* c, int id, uint64_t count, T** out) {
   }
   if (b.cap < bytes) {
     if (b.p) {
-      hipError_t e = hipFree(b.p);
+      // stream-ordered contexts free on their own stream: no device-wide wait
+      hipError_t e = c->stream_alloc ? hipFreeAsync(b.p, c->stream) : hipFree(b.p);
       (void)e;
       c->held -= b.cap;
       b.p = nullptr;
@@ -2523,7 +2533,7 @@ int need(pk_ctx* c, int id, uint64_t count, T** out) {
     if (c->budget && c->held + bytes > c->budget)
       return c->fail(PK_ENOMEM, "device budget exceeded");
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
+    hipError_t e = c->stream_alloc ? hipMallocAsync(&p, bytes, c->stream) : hipMalloc(&p, bytes);
     if (e != hipSuccess) {
       (void)hipGetLastError();
       return c->fail(PK_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
@@ -2959,7 +2969,7 @@ int count_partitioned(pk_ctx* c, Key* S, Key* S2, int32_t* C, ull* boff, uint64_
     if (lv >= 0) PK_HIP(c, hipEventRecord(c->ev[22 + 4 * lv], st));
     c->timing.ms_scatter_form = persistent ? 1 : 2;
     if (persistent)
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_scatter_p<Key>), dim3((unsigned)pgrid), dim3(MS_NT), 0, st,
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_scatter_p<Key, MS_P_NT>), dim3((unsigned)pgrid), dim3(MS_P_NT), 0, st,
                          Sin, boff, tp, (int)B, dpos, db, O2, Sout);
     else
       hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_scatter<Key>), dim3((unsigned)ntiles), dim3(MS_NT), 0, st,
@@ -3476,6 +3486,146 @@ int node_share_impl(pk_ctx* c, int P, int rank) {
 
 }  // namespace
 
+// Host input to the device through pinned staging: PIN_THREADS host threads
+// each take a contiguous slice and copy it PIN_CHUNK bytes at a time into
+// one of their two pinned buffers, then queue the DMA copy on the context
+// stream; a buffer is refilled once the event after its last copy has
+// completed.  A DMA copy from pageable memory runs at about half the rate of
+// one from pinned memory (27 against 56 GB/s measured on an MI355X host), so
+// the staging pays for its host memcpy.  Small inputs and pinned inputs take
+// one direct copy.
+constexpr int PIN_THREADS = 4;
+constexpr uint64_t PIN_CHUNK = 8ull << 20;
+constexpr uint64_t PIN_MIN = 16ull << 20;
+
+// Pinned staging shared by host-memory input and file input: fill(buf, off,
+// len) puts bytes [off, off + len) of the input into the pinned buffer
+// (returns false on a read error).
+template <class Fill>
+static int h2d_pinned(pk_ctx* c, uint8_t* dst, uint64_t n, Fill&& fill, bool* read_failed);
+static bool pin_buffers(pk_ctx* c);
+
+static int h2d_staged(pk_ctx* c, uint8_t* dst, const char* src, uint64_t n) {
+  // pinned (page-locked) input is copied as it is
+  hipPointerAttribute_t attr{};
+  const bool pinned = hipPointerGetAttributes(&attr, src) == hipSuccess && attr.type == hipMemoryTypeHost;
+  (void)hipGetLastError();  // an unregistered pointer may leave an error behind
+  if (n < PIN_MIN || pinned) {
+    PK_HIP(c, hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream));
+    return PK_OK;
+  }
+  if (!pin_buffers(c)) {
+    // no pinned memory to be had: one direct copy
+    PK_HIP(c, hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream));
+    return PK_OK;
+  }
+  bool bad = false;
+  return h2d_pinned(
+      c, dst, n,
+      [src](void* buf, uint64_t off, uint64_t len) {
+        std::memcpy(buf, src + off, len);
+        return true;
+      },
+      &bad);
+}
+
+static bool pin_buffers(pk_ctx* c) {
+  bool ready = true;
+  for (int i = 0; i < 2 * PIN_THREADS && ready; i++) {
+    if (!c->pin[i] && hipHostMalloc(&c->pin[i], PIN_CHUNK, hipHostMallocDefault) != hipSuccess) {
+      c->pin[i] = nullptr;
+      ready = false;
+    }
+    if (ready && !c->pin_ev[i] && hipEventCreateWithFlags(&c->pin_ev[i], hipEventDisableTiming) != hipSuccess) {
+      c->pin_ev[i] = nullptr;
+      ready = false;
+    }
+  }
+  if (!ready) (void)hipGetLastError();
+  return ready;
+}
+
+template <class Fill>
+static int h2d_pinned(pk_ctx* c, uint8_t* dst, uint64_t n, Fill&& fill, bool* read_failed) {
+  *read_failed = false;
+  if (!pin_buffers(c)) return c->fail(PK_ENOMEM, "pinned staging buffers could not be allocated");
+  const uint64_t slice = ((n + PIN_THREADS - 1) / PIN_THREADS + 63) & ~63ull;
+  hipError_t err[PIN_THREADS];
+  bool rerr[PIN_THREADS];
+  auto work = [&](int t) {
+    err[t] = hipSuccess;
+    rerr[t] = false;
+    const uint64_t lo = std::min<uint64_t>(n, (uint64_t)t * slice), hi = std::min<uint64_t>(n, lo + slice);
+    hipError_t e = hipSetDevice(c->device);
+    int k = 0;
+    for (uint64_t off = lo; off < hi && e == hipSuccess; off += PIN_CHUNK, k ^= 1) {
+      const int b = 2 * t + k;
+      const uint64_t len = std::min<uint64_t>(PIN_CHUNK, hi - off);
+      if (c->pin_used[b]) e = hipEventSynchronize(c->pin_ev[b]);
+      if (e != hipSuccess) break;
+      if (!fill(c->pin[b], off, len)) {
+        rerr[t] = true;
+        break;
+      }
+      e = hipMemcpyAsync(dst + off, c->pin[b], len, hipMemcpyHostToDevice, c->stream);
+      if (e == hipSuccess) e = hipEventRecord(c->pin_ev[b], c->stream);
+      c->pin_used[b] = true;
+    }
+    err[t] = e;
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < PIN_THREADS; t++) th.emplace_back(work, t);
+  work(0);
+  for (auto& x : th) x.join();
+  for (int t = 0; t < PIN_THREADS; t++) {
+    if (rerr[t]) *read_failed = true;
+    if (err[t] != hipSuccess) return c->hip_fail(err[t], "staged host-to-device copy");
+  }
+  return PK_OK;
+}
+
+// ---- File input (pk_count_file) ----------------------------------------------
+// The reference reads its rank's byte range of the FASTA file with MPI-IO
+// (DivideReads, src/input_reading.cpp:69-243); one GPU takes the whole file:
+// PIN_THREADS host threads pread their slices chunk by chunk into pinned
+// buffers while the DMA engine drains the previous chunk into HBM.
+static int file_to_device(pk_ctx* c, const char* path, const uint8_t** fa, uint64_t* nbytes) {
+  const int fd = ::open(path, O_RDONLY);
+  if (fd < 0) return c->fail(PK_EINVAL, std::string("cannot open ") + path + ": " + std::strerror(errno));
+  struct stat st {};
+  if (::fstat(fd, &st) != 0 || !S_ISREG(st.st_mode)) {
+    ::close(fd);
+    return c->fail(PK_EINVAL, std::string(path) + " is not a regular file");
+  }
+  const uint64_t n = (uint64_t)st.st_size;
+  uint8_t* d;
+  int rc = need(c, B_FASTA, n + 64, &d);
+  bool bad = false;
+  if (rc == PK_OK && n) {
+    rc = h2d_pinned(
+        c, d, n,
+        [fd](void* buf, uint64_t off, uint64_t len) {
+          uint64_t got = 0;
+          while (got < len) {
+            const ssize_t r = ::pread(fd, (char*)buf + got, len - got, (off_t)(off + got));
+            if (r <= 0) {
+              if (r < 0 && errno == EINTR) continue;
+              return false;
+            }
+            got += (uint64_t)r;
+          }
+          return true;
+        },
+        &bad);
+  }
+  ::close(fd);
+  if (rc) return rc;
+  if (bad) return c->fail(PK_EINVAL, std::string("read error in ") + path);
+  *fa = d;
+  *nbytes = n;
+  return PK_OK;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI.
 
@@ -3513,7 +3663,8 @@ void pk_ctx_destroy(pk_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& b : c->buf)
-    if (b.p) (void)hipFree(b.p);
+    if (b.p) (void)(c->stream_alloc ? hipFreeAsync(b.p, c->stream) : hipFree(b.p));
+  if (c->stream_alloc && c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (int i = 0; i < 8; i++) {
@@ -3541,15 +3692,32 @@ int pk_ctx_set_tuning(pk_ctx* c, const pk_tuning* t) {
   return PK_OK;
 }
 
+int pk_ctx_set_alloc_mode(pk_ctx* c, int mode) {
+  if (!c || (mode != PK_ALLOC_DEVICE && mode != PK_ALLOC_STREAM_ORDERED)) return PK_EINVAL;
+  const bool want = mode == PK_ALLOC_STREAM_ORDERED;
+  if (want == c->stream_alloc) return PK_OK;
+  PK_HIP(c, hipSetDevice(c->device));
+  if (want) {
+    int pools = 0;
+    if (hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, c->device) != hipSuccess || !pools)
+      return c->fail(PK_EINVAL, "the device has no stream-ordered memory pools");
+  }
+  // buffers are freed the way they were allocated
+  PK_TRY(pk_ctx_release_buffers(c));
+  c->stream_alloc = want;
+  return PK_OK;
+}
+
 int pk_ctx_release_buffers(pk_ctx* c) {
   if (!c) return PK_EINVAL;
   PK_HIP(c, hipSetDevice(c->device));
   PK_HIP(c, hipStreamSynchronize(c->stream));
   for (auto& b : c->buf) {
-    if (b.p) PK_HIP(c, hipFree(b.p));
+    if (b.p) PK_HIP(c, c->stream_alloc ? hipFreeAsync(b.p, c->stream) : hipFree(b.p));
     b.p = nullptr;
     b.cap = 0;
   }
+  if (c->stream_alloc) PK_HIP(c, hipStreamSynchronize(c->stream));
   c->held = 0;
   c->have_table = c->have_kept = c->have_nodes = false;
   c->have_norm = false;
@@ -3572,6 +3740,27 @@ int pk_count(pk_ctx* c, const char* fasta, uint64_t nbytes, int k, int fasta_on_
   const uint8_t* fa;
   PK_TRY(fasta_to_device(c, fasta, nbytes, fasta_on_device, &fa));
   int rc = (k <= 32) ? count_impl<uint64_t>(c, fa, nbytes, k) : count_impl<u128>(c, fa, nbytes, k);
+  if (rc) return rc;
+  out->n = c->U;
+  out->words = c->words;
+  out->k = c->k;
+  out->keys = reinterpret_cast<const uint64_t*>(c->buf[B_TK].p);
+  out->counts = reinterpret_cast<const int32_t*>(c->buf[B_TC].p);
+  return PK_OK;
+}
+
+// ---- File input (pk_count_file): file_to_device above the C ABI --------------
+
+int pk_count_file(pk_ctx* c, const char* path, int k, pk_table* out) {
+  if (!c || !out || !path) return PK_EINVAL;
+  if (k < 8 || k > 64) return c->fail(PK_EINVAL, "k must be in [8, 64]");
+  c->timing = pk_timing{};
+  c->pend_count = c->pend_split = false;
+  PK_HIP(c, hipSetDevice(c->device));
+  const uint8_t* fa;
+  uint64_t n = 0;
+  PK_TRY(file_to_device(c, path, &fa, &n));
+  int rc = (k <= 32) ? count_impl<uint64_t>(c, fa, n, k) : count_impl<u128>(c, fa, n, k);
   if (rc) return rc;
   out->n = c->U;
   out->words = c->words;
@@ -3648,71 +3837,6 @@ int pk_count_noalloc(pk_ctx* c, const char* fasta_dev, uint64_t nbytes, int k, u
   return PK_OK;
 }
 
-// Host input to the device through pinned staging: PIN_THREADS host threads
-// each take a contiguous slice and copy it PIN_CHUNK bytes at a time into
-// one of their two pinned buffers, then queue the DMA copy on the context
-// stream; a buffer is refilled once the event after its last copy has
-// completed.  A DMA copy from pageable memory runs at about half the rate of
-// one from pinned memory (27 against 56 GB/s measured on an MI355X host), so
-// the staging pays for its host memcpy.  Small inputs and pinned inputs take
-// one direct copy.
-constexpr int PIN_THREADS = 4;
-constexpr uint64_t PIN_CHUNK = 8ull << 20;
-constexpr uint64_t PIN_MIN = 16ull << 20;
-
-static int h2d_staged(pk_ctx* c, uint8_t* dst, const char* src, uint64_t n) {
-  // pinned (page-locked) input is copied as it is
-  hipPointerAttribute_t attr{};
-  const bool pinned = hipPointerGetAttributes(&attr, src) == hipSuccess && attr.type == hipMemoryTypeHost;
-  (void)hipGetLastError();  // an unregistered pointer may leave an error behind
-  if (n < PIN_MIN || pinned) {
-    PK_HIP(c, hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream));
-    return PK_OK;
-  }
-  bool ready = true;
-  for (int i = 0; i < 2 * PIN_THREADS && ready; i++) {
-    if (!c->pin[i] && hipHostMalloc(&c->pin[i], PIN_CHUNK, hipHostMallocDefault) != hipSuccess) {
-      c->pin[i] = nullptr;
-      ready = false;
-    }
-    if (ready && !c->pin_ev[i] && hipEventCreateWithFlags(&c->pin_ev[i], hipEventDisableTiming) != hipSuccess) {
-      c->pin_ev[i] = nullptr;
-      ready = false;
-    }
-  }
-  if (!ready) {
-    // no pinned memory to be had: one direct copy
-    (void)hipGetLastError();
-    PK_HIP(c, hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream));
-    return PK_OK;
-  }
-  const uint64_t slice = ((n + PIN_THREADS - 1) / PIN_THREADS + 63) & ~63ull;
-  hipError_t err[PIN_THREADS];
-  auto work = [&](int t) {
-    err[t] = hipSuccess;
-    const uint64_t lo = std::min<uint64_t>(n, (uint64_t)t * slice), hi = std::min<uint64_t>(n, lo + slice);
-    hipError_t e = hipSetDevice(c->device);
-    int k = 0;
-    for (uint64_t off = lo; off < hi && e == hipSuccess; off += PIN_CHUNK, k ^= 1) {
-      const int b = 2 * t + k;
-      const uint64_t len = std::min<uint64_t>(PIN_CHUNK, hi - off);
-      if (c->pin_used[b]) e = hipEventSynchronize(c->pin_ev[b]);
-      if (e != hipSuccess) break;
-      std::memcpy(c->pin[b], src + off, len);
-      e = hipMemcpyAsync(dst + off, c->pin[b], len, hipMemcpyHostToDevice, c->stream);
-      if (e == hipSuccess) e = hipEventRecord(c->pin_ev[b], c->stream);
-      c->pin_used[b] = true;
-    }
-    err[t] = e;
-  };
-  std::vector<std::thread> th;
-  for (int t = 1; t < PIN_THREADS; t++) th.emplace_back(work, t);
-  work(0);
-  for (auto& x : th) x.join();
-  for (int t = 0; t < PIN_THREADS; t++)
-    if (err[t] != hipSuccess) return c->hip_fail(err[t], "staged host-to-device copy");
-  return PK_OK;
-}
 
 static int fasta_to_device(pk_ctx* c, const char* fasta, uint64_t nbytes, int fasta_on_device,
                            const uint8_t** fa) {

@@ -151,8 +151,15 @@ void pk_server::run(int slot, Job* j) {
                  o4 = align256(o3 + kt.n * w), o5 = align256(o4 + kt.n * 4),
                  end = align256(o5 + mt.n * sizeof(pk_mnode));
   hipStream_t st = (hipStream_t)pk_ctx_stream(c);
+  // the job's kernels finish before its result block is allocated
+  hipError_t e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    r.status = PK_EHIP;
+    std::snprintf(r.message, sizeof(r.message), "job failed: %s", hipGetErrorString(e));
+    return;
+  }
   void* mem = nullptr;
-  hipError_t e = pools ? hipMallocAsync(&mem, end ? end : 256, st) : hipMalloc(&mem, end ? end : 256);
+  e = pools ? hipMallocAsync(&mem, end ? end : 256, st) : hipMalloc(&mem, end ? end : 256);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     r.status = PK_ENOMEM;
@@ -244,6 +251,19 @@ int pk_server_create(pk_server** out, int device, int n_slots, uint64_t hbm_budg
   int pools = 0;
   if (hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, device) == hipSuccess)
     s->pools = pools != 0;
+  if (s->pools) {
+    // results freed back to the device's default pool stay there for the
+    // next job's results instead of being returned to the system at the
+    // next synchronisation (which would make every later result
+    // allocation map fresh memory)
+    hipMemPool_t pool = nullptr;
+    uint64_t keep = UINT64_MAX;
+    if (hipDeviceGetDefaultMemPool(&pool, device) != hipSuccess ||
+        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) {
+      (void)hipGetLastError();
+      s->pools = false;
+    }
+  }
   if (hipStreamCreateWithFlags(&s->free_stream, hipStreamNonBlocking) != hipSuccess) {
     delete s;
     return PK_EHIP;
@@ -251,7 +271,12 @@ int pk_server_create(pk_server** out, int device, int n_slots, uint64_t hbm_budg
   for (int i = 0; i < n_slots; i++) {
     pk_ctx* c = nullptr;
     int rc = pk_ctx_create(&c, device, 0);
+    // slots grow their buffers on their own streams: a slot's growth must
+    // not wait for the other slots' work (under sustained load a
+    // device-wide wait stalled every slot for seconds)
+    if (rc == PK_OK && s->pools) rc = pk_ctx_set_alloc_mode(c, PK_ALLOC_STREAM_ORDERED);
     if (rc) {
+      if (c) pk_ctx_destroy(c);
       for (auto* x : s->ctx) pk_ctx_destroy(x);
       (void)hipStreamDestroy(s->free_stream);
       delete s;

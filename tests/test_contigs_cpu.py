@@ -129,3 +129,35 @@ def test_planted_record_mistake_changes_the_wiring(oracle_mod):
     with tempfile.TemporaryDirectory() as work:
         wired, _ = _run_filled(k, bumped, order, cov, t, work, walk=False)
     assert wired != want
+
+
+def test_mn_map_fill_time(tmp_path):
+    """Host cost of materialising the reference's MN_map from node records
+    (the filler, pakhip::fill_MN_map, in the reference-built harness): 2e6
+    synthetic records with 1-3 prefixes and suffixes each, k = 64.  Printed
+    for the design notes; asserted only to finish."""
+    k = 64
+    exe = _ref_exe(k)
+    rng = np.random.default_rng(3)
+    m = 2_000_000
+    rec = np.zeros(m, dtype=MNODE)
+    keys = np.sort(rng.integers(0, 1 << 62, m, dtype=np.int64).astype(np.uint64))
+    rec["key_lo"] = keys
+    rec["key_hi"] = keys >> np.uint64(3)
+    present = rng.random((m, 4)) < 0.45
+    present[:, 0] = True
+    rec["pre_cnt"] = np.where(present, rng.integers(1, 200, (m, 4)), 0)
+    rec["suf_cnt"] = np.where(np.roll(present, 1, axis=1), rng.integers(1, 200, (m, 4)), 0)
+    # each node's present prefixes in ascending base order, then 255
+    idx = np.argsort(~present, axis=1, kind="stable")
+    order = np.where(np.take_along_axis(present, idx, 1), idx, 255).astype(np.uint8)
+    path = tmp_path / "nodes.bin"
+    _write_records(str(path), rec, order, k)
+    env = dict(os.environ, OMP_NUM_THREADS="1", PAKREF_NODES=str(path), PAKREF_FILL_ONLY="1",
+               PAKREF_DUMP=str(tmp_path / "d"))
+    proc = subprocess.run([exe, "-f", "/dev/null", "-c", "50", "-r", "150", "-t", "21", "-n", "1000"],
+                          cwd=tmp_path, env=env, capture_output=True, text=True, timeout=600)
+    assert proc.returncode == 0, proc.stderr[-2000:]
+    line = [ln for ln in proc.stdout.splitlines() if ln.startswith("MN_map fill")]
+    assert line
+    print(line[0], f"({m} records)")

@@ -450,3 +450,48 @@ def test_initial_wiring_matches_reference(ctx, oracle_mod, name, k, cov, t):
         multi += int(n_p > 2 or n_s > 2)
     if name == "rep":
         assert multi > 0  # repeat families leave branching nodes
+
+
+def test_count_file(ctx, oracle_mod, gen_mod, tmp_path):
+    """pk_count_file (file -> pinned staging -> HBM) gives pk_count's table;
+    bad paths fail with PK_EINVAL."""
+    from pakman_plus_amd import api
+    _p("case: count from a file")
+    for name, fa in (("mid", read_fasta("syn_mid.fa.gz")),
+                     ("small", bytes(gen_mod.make_fasta(gen_mod.config_small())))):  # 33 MB: staged
+        f = tmp_path / f"{name}.fa"
+        f.write_bytes(fa)
+        for k in (32, 64):
+            ctx.count_file(str(f), k)
+            keys, counts = ctx.table_host()
+            ctx.count(fa, k)
+            k2, c2 = ctx.table_host()
+            assert np.array_equal(keys, k2) and np.array_equal(counts, c2), (name, k)
+    e = tmp_path / "empty.fa"
+    e.write_bytes(b"")
+    assert ctx.count_file(str(e), 32).n == 0
+    for bad in (str(tmp_path / "missing.fa"), str(tmp_path)):
+        with pytest.raises(api.PakHipError) as err:
+            ctx.count_file(bad, 32)
+        assert err.value.code == api.PK_EINVAL
+
+
+def test_stream_ordered_alloc_mode(oracle_mod):
+    """A context in the stream-ordered allocation mode (the server's slots)
+    counts the same tables, also while its buffers grow call by call."""
+    from pakman_plus_amd import api
+    _p("case: stream-ordered allocation mode")
+    c = api.Context(0)
+    try:
+        c.set_alloc_mode(True)
+        for name, k in (("syn_tiny.fa.gz", 32), ("syn_mid.fa.gz", 64), ("syn_rep.fa.gz", 32)):
+            fa = read_fasta(name)
+            c.count(fa, k)
+            keys, counts = c.table_host()
+            ok, oc = oracle_mod.count(fa, k)
+            assert np.array_equal(keys, ok) and np.array_equal(counts, oc), name
+        c.set_alloc_mode(False)
+        c.count(fa, k)
+        assert c.table.n == ok.shape[0]
+    finally:
+        c.close()

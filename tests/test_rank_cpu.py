@@ -79,3 +79,20 @@ def test_oracle_rank_share_matches_owner_walk(oracle_mod, k, P):
     for r in range(P):
         keys, counts = oracle_mod.count(NON_ACGT, k, nranks=P, rank=r)
         assert RU.table_counter(keys, counts) == RU.share_counter(walk, r), r
+
+
+@pytest.mark.parametrize("name,k,t", [("syn_mid.fa.gz", 32, 21), ("syn_rep.fa.gz", 64, 15), ("syn_tiny.fa.gz", 33, 21)])
+def test_pipeline_at_p_ranks(oracle_mod, name, k, t):
+    """The CPU baseline's multi-rank pipeline (the reference at P ranks, one
+    thread per rank) gives the one-rank totals on ACGT-only reads: the ranks'
+    tables partition the table, the histogram sums to the one-rank
+    histogram, and every node is built by exactly its owner."""
+    fa = read_fasta(name)
+    keys, counts = oracle_mod.count(fa, k)
+    hist, mb, keep, rc = oracle_mod.cutoff(counts, t)
+    nodes = oracle_mod.macro_nodes(keys[keep], counts[keep], k)
+    for P in (1, 2, 5):
+        r = oracle_mod.pipeline_ranks(fa, k, t, P)
+        assert (r["distinct"], r["kept"], r["nodes"], r["min_bucket"]) == (
+            keys.shape[0], int(keep.sum()), nodes["keys"].shape[0], mb), P
+        assert np.array_equal(r["hist"], hist) and r["instances"] == int(counts.sum())
