@@ -47,14 +47,14 @@ WORKLOADS = {
 }
 
 
-# stages timed by HIP events around their kernel launches -> (kernel name,
-# timing field, launches per step: None = one; a name = that timing field)
-KERNELS = {"hist": ("k_l1_hist<{key}, 0, false>", "hist_ms", None),
-           "l1_scatter": ("k_l1_scatter<{key}, 0, false>", "l1_scatter_ms", None),
-           "ms_hist": ("k_ms_hist<{key}>", "ms_hist_ms", "ms_levels"),
-           "ms_scatter": ("{ms}<{key}>", "ms_scatter_ms", "ms_levels"),
-           "sort": ("k_bucket_hash<{key}>", "sort_ms", None),
-           "compact": ("k_compact<{key}>", "compact_ms", None)}
+# single-launch stages timed by HIP events around their kernel -> (kernel
+# name, timing field); the further partition levels are added per level from
+# the per-level fields (ms_level_*), their scatter kernel by form
+KERNELS = {"hist": ("k_l1_hist<{key}, 0, false>", "hist_ms"),
+           "l1_scatter": ("k_l1_scatter<{key}, 0, false>", "l1_scatter_ms"),
+           "sort": ("k_bucket_hash<{key}>", "sort_ms"),
+           "compact": ("k_compact<{key}>", "compact_ms")}
+MS_SCATTER = {1: "k_ms_scatter_p", 2: "k_ms_scatter", 3: "k_ms_scatter_seg"}
 
 
 def alg_bytes(N, U, V, M, S, W):
@@ -74,27 +74,32 @@ def alg_bytes(N, U, V, M, S, W):
     }
 
 
+def kernel_times(tm, W):
+    """[(stage, kernel, ms per step)] of every single-launch kernel the stage
+    timings cover: the four fixed stages, then each further partition level's
+    histogram (a k_ms_hist launch unless the level's counts were folded into
+    level 1's histogram, form 3) and scatter."""
+    key = "unsigned long" if W == 8 else "unsigned __int128"
+    out = [(stage, name.format(key=key), float(tm.get(field, 0.0))) for stage, (name, field) in KERNELS.items()]
+    levels = min(int(tm.get("ms_levels", 0)), len(tm.get("ms_level_form", [])))
+    for lv in range(levels):
+        form = int(tm["ms_level_form"][lv])
+        if form != 3:
+            out.append(("ms_hist", f"k_ms_hist<{key}>", float(tm["ms_level_hist_ms"][lv])))
+        out.append(("ms_scatter", f"{MS_SCATTER[form]}<{key}>", float(tm["ms_level_scatter_ms"][lv])))
+    return [x for x in out if x[2] > 0]
+
+
 def dominant_kernel(tm, N, U, V, M, S, W, workload):
-    """The kernel stage with the largest time per step, its time per launch and
+    """The kernel with the largest time per step (each is one launch), and its
     achieved algorithmic bandwidth (alg_bytes per launch / time per launch)."""
     ab = alg_bytes(N, U, V, M, S, W)
-    best = None
-    for stage, (_, field, per) in KERNELS.items():
-        total = float(tm.get(field, 0.0))
-        launches = int(tm.get(per, 0)) if per else 1
-        if total <= 0 or launches <= 0:
-            continue
-        if best is None or total > best[1]:
-            best = (stage, total, launches)
-    stage, total, launches = best
-    per_ms = total / launches
-    key = "unsigned long" if W == 8 else "unsigned __int128"
-    ms = "k_ms_scatter_p" if int(tm.get("ms_scatter_form", 0)) == 1 else "k_ms_scatter"
+    stage, kernel, per_ms = max(kernel_times(tm, W), key=lambda x: x[2])
     achieved = ab[stage] / (per_ms * 1e-3)
-    return {"bound": "hbm", "kernel": KERNELS[stage][0].format(key=key, ms=ms),
+    return {"bound": "hbm", "kernel": kernel,
             "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": load_traffic(stage, workload),
-            "alg_bytes_per_launch": ab[stage], "avg_ms": per_ms, "launches_per_step": launches,
+            "frac": achieved / HBM_PEAK, "traffic": load_traffic(kernel, workload),
+            "alg_bytes_per_launch": ab[stage], "avg_ms": per_ms, "launches_per_step": 1,
             "share_of_step": None}
 
 
@@ -386,15 +391,16 @@ def bench_ingest(fa, k, t, steps, tun, plain_ms):
             "note": "file- and PCIe-inclusive; not the line's value"}
 
 
-def load_traffic(stage, workload):
-    """HBM bytes per launch of the stage's kernel, from the rocprofv3 PMC runs
-    (FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a plain copy of
-    known size) summarised in profiles/traffic.json; None when not measured."""
+def load_traffic(kernel, workload):
+    """HBM bytes per launch of the kernel on the workload, from the rocprofv3
+    PMC runs (FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a
+    plain copy of known size) summarised in profiles/traffic.json; None when
+    not measured."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        e = json.load(f).get(f"{stage}:{workload}")
+        e = json.load(f).get(f"{kernel} @ {workload}")
     return float(e["bytes"]) if e else None
 
 
@@ -437,13 +443,17 @@ def run_workload(name, steps, warmup, t):
         step()
     keys = ("parse_ms", "hist_ms", "scatter_ms", "l1_scatter_ms", "ms_hist_ms", "ms_scatter_ms", "sort_ms",
             "split_ms", "compact_ms", "cutoff_ms", "mn_ms")
+    lkeys = ("ms_level_hist_ms", "ms_level_scatter_ms")
     acc = {x: 0.0 for x in keys}
+    lacc = {x: [0.0] * 4 for x in lkeys}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         tm = step()
         for x in keys:
             acc[x] += tm[x]
+        for x in lkeys:
+            lacc[x] = [a + b for a, b in zip(lacc[x], tm[x])]
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     N = int(tm["n_instances"])
@@ -451,7 +461,9 @@ def run_workload(name, steps, warmup, t):
     W = 8 if k <= 32 else 16
     ms_per_step = elapsed * 1e3 / steps
     avg = {x: acc[x] / steps for x in keys}
-    avg.update({"ms_levels": int(tm["ms_levels"]), "ms_scatter_form": int(tm["ms_scatter_form"])})
+    avg.update({"ms_levels": int(tm["ms_levels"]), "ms_scatter_form": int(tm["ms_scatter_form"]),
+                "ms_level_form": list(tm["ms_level_form"])})
+    avg.update({x: [v / steps for v in lacc[x]] for x in lkeys})
     ab = alg_bytes(N, U, V, M, nbytes, W)
     roof = dominant_kernel(avg, N, U, V, M, nbytes, W, name)
     roof["share_of_step"] = roof["avg_ms"] * roof["launches_per_step"] / ms_per_step
@@ -472,7 +484,12 @@ def run_workload(name, steps, warmup, t):
         "roofline": roof,
         "pipeline": {"alg_bytes": ab["pipeline"], "alg_bytes_per_instance": ab["pipeline"] / max(N, 1),
                      "frac_of_peak": ab["pipeline"] / (ms_per_step * 1e-3) / HBM_PEAK,
-                     "stage_ms": {x.replace("_ms", ""): v for x, v in avg.items()}},
+                     "stage_ms": {x.replace("_ms", ""): v for x, v in avg.items()},
+                     # every single-launch kernel timed per step, longest first
+                     "kernels": [{"kernel": kn, "ms": ms, "alg_bytes": ab[sg],
+                                  "frac": ab[sg] / (ms * 1e-3) / HBM_PEAK,
+                                  "traffic": load_traffic(kn, name)}
+                                 for sg, kn, ms in sorted(kernel_times(avg, W), key=lambda x: -x[2])]},
     }
     return out, fa, k, tun
 

@@ -102,7 +102,12 @@ typedef struct pk_timing {
   float ms_scatter_ms;  /* further partition levels: their scatter kernels, summed */
   int32_t ms_levels;    /* further partition levels of the last count (timed: first 4) */
   int32_t ms_scatter_form; /* their scatter kernel: 1 persistent blocks, 2 one block
-                              per tile (the last level's; 0 when none ran) */
+                              per tile, 3 segments of a histogram folded into
+                              level 1's (the last level's; 0 when none ran) */
+  float ms_level_hist_ms[4];    /* per further level (the first 4): its histogram
+                                   kernel, or with form 3 the scan of the folded counts */
+  float ms_level_scatter_ms[4]; /* per further level: its scatter kernel */
+  int32_t ms_level_form[4];     /* per further level: its scatter form as above */
 } pk_timing;
 
 /* Create a context on HIP device `device`.  hbm_budget_bytes = 0 means no
@@ -127,13 +132,18 @@ uint64_t pk_ctx_device_bytes(const pk_ctx* ctx);
  *   ms_scatter     form of the further partition levels' scatter: 1 =
  *                  persistent blocks (two per CU, consecutive tiles each),
  *                  2 = one block per tile (built-in: persistent once every
- *                  block gets >= 256 tiles) */
+ *                  block gets >= 256 tiles)
+ *   ms_fold        2 = never count the second partition level inside the
+ *                  level-1 histogram (built-in: counted there when two or
+ *                  more levels follow level 1 and the first two digits have
+ *                  at most 8192 values; 1 = the same) */
 #define PK_TUNING_L1_BITS_MAX 10
 typedef struct pk_tuning {
   int32_t l1_chunks;
   int32_t l1_bits;
   float bucket_factor;
   int32_t ms_scatter;
+  int32_t ms_fold;
 } pk_tuning;
 int pk_ctx_set_tuning(pk_ctx* ctx, const pk_tuning* t);
 

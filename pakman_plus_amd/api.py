@@ -49,13 +49,16 @@ class _Timing(ctypes.Structure):
                 ("bucket_bits", ctypes.c_int32), ("n_split", ctypes.c_int32),
                 ("l1_scatter_ms", ctypes.c_float), ("normalize_ms", ctypes.c_float),
                 ("ms_hist_ms", ctypes.c_float), ("ms_scatter_ms", ctypes.c_float),
-                ("ms_levels", ctypes.c_int32), ("ms_scatter_form", ctypes.c_int32)]
+                ("ms_levels", ctypes.c_int32), ("ms_scatter_form", ctypes.c_int32),
+                ("ms_level_hist_ms", ctypes.c_float * 4), ("ms_level_scatter_ms", ctypes.c_float * 4),
+                ("ms_level_form", ctypes.c_int32 * 4)]
 
 
 class Tuning(ctypes.Structure):
     """Launch-shape choices of the counting kernels (pk_tuning); 0 = built-in."""
     _fields_ = [("l1_chunks", ctypes.c_int32), ("l1_bits", ctypes.c_int32),
-                ("bucket_factor", ctypes.c_float), ("ms_scatter", ctypes.c_int32)]
+                ("bucket_factor", ctypes.c_float), ("ms_scatter", ctypes.c_int32),
+                ("ms_fold", ctypes.c_int32)]
 
 
 class _TuneEntry(ctypes.Structure):
@@ -278,8 +281,8 @@ class Context:
         self.n_kept = self.n_nodes = 0
 
     def set_tuning(self, l1_chunks: int = 0, l1_bits: int = 0, bucket_factor: float = 0.0,
-                   ms_scatter: int = 0):
-        t = Tuning(l1_chunks, l1_bits, bucket_factor, ms_scatter)
+                   ms_scatter: int = 0, ms_fold: int = 0):
+        t = Tuning(l1_chunks, l1_bits, bucket_factor, ms_scatter, ms_fold)
         self._check(self._lib.pk_ctx_set_tuning(self._h, ctypes.byref(t)))
 
     # -- perform_kmer_counting ------------------------------------------------
@@ -496,7 +499,11 @@ class Context:
     def timing(self) -> dict:
         t = _Timing()
         self._check(self._lib.pk_last_timing(self._h, ctypes.byref(t)))
-        return {f: getattr(t, f) for f, _ in _Timing._fields_}
+        out = {}
+        for f, _ in _Timing._fields_:
+            v = getattr(t, f)
+            out[f] = list(v) if isinstance(v, ctypes.Array) else v
+        return out
 
 
 # ---------------------------------------------------------------------------
@@ -647,7 +654,7 @@ class Server:
             arr[i] = _TuneEntry(int(e["max_bytes"]), int(e.get("key_words", 0)),
                                 Tuning(int(e.get("l1_chunks", 0)), int(e.get("l1_bits", 0)),
                                        float(e.get("bucket_factor", 0.0)),
-                                       int(e.get("ms_scatter", 0))))
+                                       int(e.get("ms_scatter", 0)), int(e.get("ms_fold", 0))))
         self._check(self._lib.pk_server_set_tuning(self._h, arr, len(entries)))
 
 

@@ -40,7 +40,7 @@ def _run(cmd):
 def build_gen() -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     out = os.path.join(LIBDIR, "libpkgen.so")
-    tmp = out + ".tmp"
+    tmp = f"{out}.{os.getpid()}.tmp"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp",
           os.path.join(CSRC, "pk_gen.cpp"), "-o", tmp])
     os.replace(tmp, out)
@@ -50,7 +50,7 @@ def build_gen() -> str:
 def build_hip() -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     out = os.path.join(LIBDIR, "libpakhip.so")
-    tmp = out + ".tmp"
+    tmp = f"{out}.{os.getpid()}.tmp"
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
     _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", *srcs, "-o", tmp])
     os.replace(tmp, out)
@@ -64,7 +64,7 @@ def build_adapter_test() -> str:
     outdir = os.path.join(ROOT, "tests", "cpp", "_bin")
     os.makedirs(outdir, exist_ok=True)
     out = os.path.join(outdir, "builder_adapter_test")
-    tmp = out + ".tmp"
+    tmp = f"{out}.{os.getpid()}.tmp"  # concurrent builders each rename their own
     _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", src, f"-L{LIBDIR}", "-lpakhip",
           "-Wl,-rpath,$ORIGIN/../../../pakman_plus_amd/_lib", "-Wl,-rpath,/opt/rocm/lib", "-o", tmp])
     os.replace(tmp, out)
@@ -77,8 +77,9 @@ def build_prefix_order_test() -> str:
     outdir = os.path.join(ROOT, "tests", "cpp", "_bin")
     os.makedirs(outdir, exist_ok=True)
     out = os.path.join(outdir, "prefix_order_test")
-    _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", src, "-o", out + ".tmp"])
-    os.replace(out + ".tmp", out)
+    tmp = f"{out}.{os.getpid()}.tmp"
+    _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", src, "-o", tmp])
+    os.replace(tmp, out)
     return out
 
 

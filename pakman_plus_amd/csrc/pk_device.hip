@@ -338,8 +338,14 @@ __global__ __launch_bounds__(1024) void k_min_reduce(const ull* in, uint64_t n, 
 // Positions of the event bytes, ascending, from the block's base blk_off.
 // The four slab counts of a thread travel through one block scan packed in
 // 16-bit fields (a slab holds at most 4096 events, so no field carries).
+// With follow = 1, bits 62-63 of each entry tell what follows the event: 1
+// a '>', 2 the end of the input, 0 anything else (the record check would
+// otherwise read that byte per record, one scattered line each).
+constexpr int NL_FLAG_SHIFT = 62;
+constexpr ull NL_POS_MASK = (1ull << NL_FLAG_SHIFT) - 1;
+
 __global__ __launch_bounds__(NL_NT) void k_nl_write(const uint8_t* fa, uint64_t n, uint8_t e0,
-                                                    uint8_t e1, const ull* blk_off, ull* nl) {
+                                                    uint8_t e1, const ull* blk_off, ull* nl, int follow) {
   const uint64_t b0 = (uint64_t)blockIdx.x * NL_PER_BLOCK + (uint64_t)threadIdx.x * 16;
   uint32_t ev[NL_SLABS], gm[NL_SLABS];
   ull packed = 0;
@@ -355,7 +361,17 @@ __global__ __launch_bounds__(NL_NT) void k_nl_write(const uint8_t* fa, uint64_t 
   for (int j = 0; j < NL_SLABS; j++) {
     ull o = slab_base + ((ex >> (16 * j)) & 0xFFFFull);
     const uint64_t pos = b0 + j * NL_SLAB;
-    for (uint32_t m = ev[j]; m; m &= m - 1) nl[o++] = pos + __ffs(m) - 1;
+    for (uint32_t m = ev[j]; m; m &= m - 1) {
+      const int i = __ffs(m) - 1;
+      const uint64_t p = pos + i;
+      ull f = 0;
+      if (follow) {
+        // the next byte: in this thread's 16 bytes, else the next chunk's
+        // first (the neighbour's load, usually still in cache)
+        f = p + 1 >= n ? 2ull : i < 15 ? (ull)((gm[j] >> (i + 1)) & 1u) : (ull)(fa[p + 1] == '>');
+      }
+      nl[o++] = p | (f << NL_FLAG_SHIFT);
+    }
     slab_base += (tot >> (16 * j)) & 0xFFFFull;
   }
 }
@@ -636,26 +652,31 @@ __global__ void k_win_keys(const uint8_t* fa, const ull* rstart, const uint32_t*
 
 // Records (kmer_counting.cpp:2920-2945).  Candidate r uses header newline
 // nl[j0+2r] and sequence newline nl[j0+2r+1] (or the end of the buffer).
-__global__ void k_reads(const uint8_t* fa, uint64_t n, const ull* nl, uint64_t m, uint64_t p0,
+__global__ void k_reads(uint64_t n, const ull* nl, uint64_t m, uint64_t p0,
                         int K, uint64_t ncand, ull* rstart, uint32_t* rlen, uint32_t* kc,
                         ull* first_unproc, ull* first_err, ull* too_long) {
   const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= ncand) return;
-  // j0 = first newline at or after p0
-  uint64_t lo = 0, hi = m;
-  while (lo < hi) {
-    uint64_t mid = (lo + hi) >> 1;
-    if (nl[mid] < p0) lo = mid + 1; else hi = mid;
+  // j0 = first newline at or after p0 (one search per block)
+  __shared__ uint64_t sj0;
+  if (threadIdx.x == 0) {
+    uint64_t lo = 0, hi = m;
+    while (lo < hi) {
+      uint64_t mid = (lo + hi) >> 1;
+      if ((nl[mid] & NL_POS_MASK) < p0) lo = mid + 1; else hi = mid;
+    }
+    sj0 = lo;
   }
-  const uint64_t j0 = lo;
+  __syncthreads();
+  if (r >= ncand) return;
+  const uint64_t j0 = sj0;
   const uint64_t h = j0 + 2 * r;
   bool processed = false;
   if (p0 < n && h < m) {
-    const uint64_t hdr_nl = nl[h];
+    const uint64_t hdr_nl = nl[h] & NL_POS_MASK;
     processed = hdr_nl + 1 + (uint64_t)K <= n;
     if (processed) {
       const uint64_t s = hdr_nl + 1;
-      const uint64_t e = (h + 1 < m) ? nl[h + 1] : n;
+      const uint64_t e = (h + 1 < m) ? nl[h + 1] & NL_POS_MASK : n;
       const uint64_t len = e - s;
       rstart[r] = s;
       if (len > 0xFFFFFFF0ull) {
@@ -677,10 +698,7 @@ __global__ void k_reads(const uint8_t* fa, uint64_t n, const ull* nl, uint64_t m
   // assert(ptr[p] == '>') at the start of record r >= 1 when it is reached
   if (r >= 1 && p0 < n) {
     const uint64_t hp = j0 + 2 * r - 1;
-    if (hp < m) {
-      const uint64_t q = nl[hp] + 1;
-      if (q < n && fa[q] != '>') atomicMin(first_err, (ull)r);
-    }
+    if (hp < m && (nl[hp] >> NL_FLAG_SHIFT) == 0) atomicMin(first_err, (ull)r);
   }
 }
 
@@ -727,6 +745,8 @@ constexpr int L1_MAX_BITS = PK_TUNING_L1_BITS_MAX;  // level-1 digit width (dire
 constexpr int L1_DEFAULT_BITS = 8;  // wider digits keep more partial lines open
 constexpr int MS_MAX_BITS = 8;   // further levels (tile split into runs)
 constexpr int MS_TIMED_LEVELS = 4;  // further levels with their own timing events
+static_assert(sizeof(pk_timing::ms_level_form) / sizeof(int32_t) == MS_TIMED_LEVELS, "per-level timing slots");
+constexpr uint64_t L2_FOLD_MAX = 8192;  // (bucket, digit) counters of a folded level-2 histogram
 
 template <class Key>
 struct TileCfg {
@@ -773,6 +793,21 @@ struct L1Scatter {
   // histogram blocks per scatter block (their output runs per digit are
   // adjacent in the digit-major scan)
   static constexpr int GF = main && sizeof(Key) == 8 ? 2 : 1;
+};
+
+// Level-2 histogram folded into the level-1 histogram (large inputs with
+// three or more partition levels): the level-2 "segments" are runs of the
+// level-1 output, segment (a, g) = the keys of level-1 bucket a that the
+// histogram blocks [g * spb, (g + 1) * spb) produced (spb a multiple of the
+// histogram blocks per scatter block, so a segment is whole scatter-block
+// runs, contiguous in the digit-major layout); the level-1 histogram kernel
+// counts them per second digit s into H2[(a, s) * nsb + g], so no pass over
+// the written keys is needed to count them.
+struct L2Fold {
+  int d2;         // second-digit bits (0: no fold)
+  int spb;        // histogram blocks per segment
+  uint64_t nsb;   // segments per level-1 bucket
+  uint32_t* H2;   // (2^(d1+d2)) x nsb counts, zeroed before the launch
 };
 
 struct ChunkLds {
@@ -988,14 +1023,18 @@ template <class Key, int ENC, bool OWN>
 __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n, const ull* rstart,
                                                    const uint32_t* rlen, uint64_t R, int K, int KB,
                                                    Key mask, int d1, int rpb, int sb, OwnCfg cf,
-                                                   uint32_t* H) {
+                                                   uint32_t* H, L2Fold f2) {
   __shared__ ChunkLds C;
-  // dynamic LDS: plane A, plane B, 2^d1 digit counts
+  // dynamic LDS: plane A, plane B, 2^d1 digit counts (then, with f2.d2 > 0,
+  // 2^(d1 + d2) counts of the first two digits)
   extern __shared__ ull l1_dyn[];
   const Stage sg{l1_dyn, reinterpret_cast<uint32_t*>(l1_dyn + sb / 32), sb};
   uint32_t* h = sg.B + sb / 32;
   const int nb = 1 << d1;
+  uint32_t* h2 = h + nb;
+  const int nb2 = f2.d2 ? nb << f2.d2 : 0;
   for (int i = threadIdx.x; i < nb; i += L1_NT) h[i] = 0;
+  for (int i = threadIdx.x; i < nb2; i += L1_NT) h2[i] = 0;
   const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
   const uint64_t r1 = r0 + rpb < R ? r0 + rpb : R;
   const int bpos = KB - d1;
@@ -1008,10 +1047,29 @@ __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n
       Key km = 0;
       OwnState os{0, 0, 0, 0};
       walk_read<Key, ENC, OWN>(C, sg, fa, rstart[r], rlen[r], i, km, os, cf, K, mask, 0x7fffffff,
-                               [&](int, Key x) { atomicAdd(&h[key_digit<Key>(x, bpos, d1)], 1u); });
+                               [&](int, Key x) {
+                                 // one LDS atomic per key: with the fold the
+                                 // level-1 counts are sums of h2's rows
+                                 if (nb2) atomicAdd(&h2[key_digit<Key>(x, bpos - f2.d2, d1 + f2.d2)], 1u);
+                                 else atomicAdd(&h[key_digit<Key>(x, bpos, d1)], 1u);
+                               });
     }
     c0 += m;
     __syncthreads();
+  }
+  if (nb2) {
+    // h[a] = sum of h2[a, 0 .. 2^d2): one wave per bucket, lanes over s
+    const int lane = (int)threadIdx.x & 63, per = 1 << f2.d2;
+    for (int a = (int)threadIdx.x >> 6; a < nb; a += L1_NT / 64) {
+      uint32_t v = 0;
+      for (int j = lane; j < per; j += 64) v += h2[(a << f2.d2) + j];
+      for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
+      if (lane == 0) h[a] = v;
+    }
+    __syncthreads();
+    // (bucket, sub-digit, segment) counts of the level-2 segments
+    for (int i = threadIdx.x; i < nb2; i += L1_NT)
+      if (h2[i]) atomicAdd(&f2.H2[(ull)i * f2.nsb + blockIdx.x / f2.spb], h2[i]);
   }
   for (int i = threadIdx.x; i < nb; i += L1_NT) H[(ull)i * gridDim.x + blockIdx.x] = h[i];
 }
@@ -1296,6 +1354,144 @@ __global__ __launch_bounds__(NT, NT >= 1024 ? 8 : 4) void k_ms_scatter_p(const K
   }
 }
 
+// Level 2 over the segments of the folded histogram (L2Fold): segment j =
+// (a, g), a = j / nsb, holds [O1[a * nblk + g * spb], O1[a * nblk +
+// min((g + 1) * spb, nblk)]) of the level-1 array; its run of sub-digit s
+// starts at O2[((a << dbits) + s) * nsb + g] (the scan of H2).  Block i takes
+// the segments that start in its 1/G share of the keys and sorts them tile by
+// tile (TQ keys) as k_ms_scatter_p does, the next tile's keys loaded ahead;
+// within a segment each tile's runs start where the previous tile's ended.
+__device__ __forceinline__ void seg_bounds(const ull* O1, uint64_t nblk, int spb, uint64_t nsb, ull j, ull& st,
+                                           ull& en) {
+  const ull a = j / nsb, sb = j - a * nsb;
+  const ull lo = sb * (ull)spb;
+  const ull hi = lo + (ull)spb < nblk ? lo + (ull)spb : nblk;
+  st = O1[a * nblk + lo];
+  en = O1[a * nblk + hi];
+}
+
+__device__ __forceinline__ ull seg_first_at(const ull* O1, uint64_t nblk, int spb, uint64_t nsb, ull nseg, ull k) {
+  ull lo = 0, hi = nseg;
+  while (lo < hi) {
+    const ull mid = (lo + hi) >> 1;
+    ull st, en;
+    seg_bounds(O1, nblk, spb, nsb, mid, st, en);
+    if (st < k) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+template <class Key, int NT>
+__global__ __launch_bounds__(NT, 4) void k_ms_scatter_seg(const Key* Sin, const ull* O1, uint64_t nblk, int spb,
+                                                          uint64_t nsb, int nb1, ull total, int dpos, int dbits,
+                                                          const ull* O2, Key* Sout) {
+  constexpr int TQ = MS_NT * TileCfg<Key>::Q;
+  constexpr int Q = TQ / NT;
+  __shared__ TileLds<Key, MS_NT> L;
+  const ull nseg = (ull)nb1 * nsb;
+  const ull k0 = total * (ull)blockIdx.x / gridDim.x;
+  const ull k1 = total * ((ull)blockIdx.x + 1) / gridDim.x;
+  ull j = seg_first_at(O1, nblk, spb, nsb, nseg, k0);
+  const ull j1 = blockIdx.x + 1 == gridDim.x ? nseg : seg_first_at(O1, nblk, spb, nsb, nseg, k1);
+  const int ns = 1 << dbits;
+  const int tid = (int)threadIdx.x;
+  // first non-empty segment
+  ull st = 0, en = 0;
+  while (j < j1) {
+    seg_bounds(O1, nblk, spb, nsb, j, st, en);
+    if (en > st) break;
+    j++;
+  }
+  if (j >= j1) return;
+  auto base_of = [&](ull jj) -> ull {
+    const ull a = jj / nsb, sb = jj - a * nsb;
+    return O2[((a << dbits) + (ull)tid) * nsb + sb];
+  };
+  ull pos = st;
+  int m = (int)(en - pos < (ull)TQ ? en - pos : (ull)TQ);
+  Key kr[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int idx = tid + q * NT;
+    kr[q] = idx < m ? Sin[pos + idx] : Key(0);
+  }
+  ull bs = tid < ns ? base_of(j) : 0ull;
+  for (;;) {
+    // the next tile: further in this segment, or the next non-empty segment
+    ull jn = j, posn = pos + (ull)m, enn = en;
+    bool more = true;
+    if (posn >= en) {
+      jn = j + 1;
+      ull s2 = 0, e2 = 0;
+      while (jn < j1) {
+        seg_bounds(O1, nblk, spb, nsb, jn, s2, e2);
+        if (e2 > s2) break;
+        jn++;
+      }
+      more = jn < j1;
+      posn = s2;
+      enn = e2;
+    }
+    Key kn[Q] = {};
+    ull bn = 0;
+    int mn = 0;
+    if (more) {
+      mn = (int)(enn - posn < (ull)TQ ? enn - posn : (ull)TQ);
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        const int idx = tid + q * NT;
+        kn[q] = idx < mn ? Sin[posn + idx] : Key(0);
+      }
+      if (tid < ns && jn != j) bn = base_of(jn);
+    }
+    if (tid < ns) L.h[tid] = 0;
+    const int np = m > tid ? (m - tid - 1) / NT + 1 : 0;
+    __syncthreads();
+    uint32_t rk[(Q + 1) / 2] = {};
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+      if (q < np) rk[q / 2] |= atomicAdd(&L.h[key_digit<Key>(kr[q], dpos, dbits)], 1u) << (16 * (q & 1));
+    __syncthreads();
+    const uint32_t v = tid < ns ? L.h[tid] : 0u;
+    if (more && jn == j) bn = bs + v;
+    const ull ex = block_excl_scan<NT>(v);
+    if (tid < ns) {
+      L.tcur[tid] = (uint32_t)ex;
+      L.delta[tid] = (long long)bs - (long long)ex;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      if (q < np) {
+        const uint32_t d = key_digit<Key>(kr[q], dpos, dbits);
+        L.T1[L.tcur[d] + ((rk[q / 2] >> (16 * (q & 1))) & 0xFFFFu)] = kr[q];
+      }
+    }
+    __syncthreads();
+    for (int p = tid; p < m; p += NT) {
+      const Key w = L.T1[p];
+      Sout[L.delta[key_digit<Key>(w, dpos, dbits)] + p] = w;
+    }
+    __syncthreads();
+    if (!more) break;
+#pragma unroll
+    for (int q = 0; q < Q; q++) kr[q] = kn[q];
+    bs = bn;
+    j = jn;
+    pos = posn;
+    en = enn;
+    m = mn;
+  }
+}
+
+// boff2[(a << dbits) | s] = O2[((a << dbits) | s) * nsb] (the first segment's
+// base), boff2[end] = total
+__global__ void k_seg_boff(const ull* O2, uint64_t nsb, ull n2, ull total, ull* boff2) {
+  const ull g = (ull)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n2) boff2[g] = O2[g * nsb];
+  if (g == n2) boff2[n2] = total;
+}
+
 // ---------------------------------------------------------------------------
 // Per-bucket count: LDS hash table of the bucket's distinct keys (64-bit CAS,
 // linear probing; duplicates cost one atomic add), then a counting sort of
@@ -1316,12 +1512,14 @@ struct HashCfg<uint64_t> {
   static constexpr int T = 2048;      // table slots
   static constexpr int LIMIT = 1536;  // distinct keys per task (load 0.75)
   static constexpr int SUB = 9;       // sub-digit bits of the ordering pass
+  static constexpr int BLOCKS = 6;    // blocks per CU the LDS allows (register cap follows)
 };
 template <>
 struct HashCfg<u128> {
   static constexpr int T = 1024;
   static constexpr int LIMIT = 768;
   static constexpr int SUB = 8;
+  static constexpr int BLOCKS = 5;
 };
 // tasks with more instances than this are split without trying
 constexpr uint64_t TASK_MAX_INSTANCES = 1u << 18;
@@ -1408,7 +1606,10 @@ struct HashTable<u128> {
       hc[i] = 0;
     }
   }
-  __device__ __forceinline__ uint32_t home(ull f) const { return (uint32_t)(f >> 40) & (T - 1); }
+  // home slot from the fingerprint's top bits, scaled to T (T need not be a
+  // power of two); probes step to the next slot, wrapping
+  __device__ __forceinline__ uint32_t home(ull f) const { return (uint32_t)(((f >> 32) * (ull)T) >> 32); }
+  __device__ __forceinline__ static uint32_t next(uint32_t slot) { return slot + 1 == (uint32_t)T ? 0u : slot + 1; }
   // a slot changes only once, from EMPTY to its fingerprint: a plain read
   // finds a repeated key or steps over another one without an atomic, and
   // only an empty slot is claimed by CAS (k = 64 bacterial: 2.73 -> 2.56 ms)
@@ -1424,21 +1625,24 @@ struct HashTable<u128> {
     }
     return false;
   }
-  __device__ __forceinline__ bool insert(u128 k, const volatile int* stop) {
+  // the slot that counts k, or -1 (table full, stopped, or a fingerprint
+  // equal to EMPTY)
+  __device__ __forceinline__ int insert_at(u128 k, const volatile int* stop) {
     const ull f = fp(k);
-    if (f == EMPTY) return false;
+    if (f == EMPTY) return -1;
     uint32_t slot = home(f);
     for (int probe = 0; probe < STOP_AFTER; probe++) {
-      if (try_slot(slot, f, k)) return true;
-      slot = (slot + 1) & (T - 1);
+      if (try_slot(slot, f, k)) return (int)slot;
+      slot = next(slot);
     }
     for (int probe = STOP_AFTER; probe < HASH_PROBES; probe++) {
-      if ((probe & 7) == 0 && *stop) return false;
-      if (try_slot(slot, f, k)) return true;
-      slot = (slot + 1) & (T - 1);
+      if ((probe & 7) == 0 && *stop) return -1;
+      if (try_slot(slot, f, k)) return (int)slot;
+      slot = next(slot);
     }
-    return false;
+    return -1;
   }
+  __device__ __forceinline__ bool insert(u128 k, const volatile int* stop) { return insert_at(k, stop) >= 0; }
   // after a barrier: does key k sit in its fingerprint's slot?
   __device__ __forceinline__ bool verify(u128 k) const {
     const ull f = fp(k);
@@ -1446,7 +1650,7 @@ struct HashTable<u128> {
     for (int probe = 0; probe < T; probe++) {
       if (hf[slot] == f) return hkf[slot] == k;
       if (hf[slot] == EMPTY) return false;
-      slot = (slot + 1) & (T - 1);
+      slot = next(slot);
     }
     return false;
   }
@@ -1495,7 +1699,7 @@ struct HashLds<u128> {
 constexpr int MAX_PASS_BITS = 4;
 
 template <class Key>
-__global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int32_t* C,
+__global__ __launch_bounds__(BS_NT, HashCfg<Key>::BLOCKS) void k_bucket_hash(const Key* S, Key* R, int32_t* C,
                                                        const ull* tstart, const ull* tcount,
                                                        uint64_t ntasks, int K, int fixed,
                                                        long long* tu, ull* ovf_count, ull* ovf_list) {
@@ -1537,6 +1741,7 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
       // latency is paid once per batch, not once per insert; a task of one
       // batch keeps them there for the fingerprint check below
       Key kb[HB];
+      int ksl[HB];  // u128: the slot each key of the (last) batch landed in
       for (uint64_t b = 0; b < n; b += (uint64_t)BS_NT * HB) {
 #pragma unroll
         for (int j = 0; j < HB; j++) {
@@ -1547,13 +1752,23 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
 #pragma unroll
         for (int j = 0; j < HB; j++) {
           const uint64_t idx = b + threadIdx.x + (uint64_t)j * BS_NT;
-          if (idx < n && (int)key_digit<Key>(kb[j], rem - s, s) == pass && !ht.insert(kb[j], &over))
-            over = 1;
+          if (idx < n && (int)key_digit<Key>(kb[j], rem - s, s) == pass) {
+            if constexpr (sizeof(Key) == 16) {
+              ksl[j] = ht.insert_at(kb[j], &over);
+              if (ksl[j] < 0) over = 1;
+            } else if (!ht.insert(kb[j], &over)) {
+              over = 1;
+            }
+          }
         }
       }
       __syncthreads();
       if constexpr (sizeof(Key) == 16) {
         if (!over) {
+          // a key counted in a slot whose stored key differs shares its
+          // fingerprint with that key: the task goes to the split path.  A
+          // one-batch task checks the slots its inserts returned; a longer
+          // one reloads its keys and finds their slots again.
           const bool one_batch = n <= (uint64_t)BS_NT * HB;
           for (uint64_t b = 0; b < n; b += (uint64_t)BS_NT * HB) {
             if (!one_batch) {
@@ -1566,7 +1781,8 @@ __global__ __launch_bounds__(BS_NT) void k_bucket_hash(const Key* S, Key* R, int
 #pragma unroll
             for (int j = 0; j < HB; j++) {
               const uint64_t idx = b + threadIdx.x + (uint64_t)j * BS_NT;
-              if (idx < n && (int)key_digit<Key>(kb[j], rem - s, s) == pass && !ht.verify(kb[j]))
+              if (idx < n && (int)key_digit<Key>(kb[j], rem - s, s) == pass &&
+                  !(one_batch ? ht.key(ksl[j]) == kb[j] : ht.verify(kb[j])))
                 over = 2;
             }
           }
@@ -2428,7 +2644,7 @@ enum BufId {
   B_FASTA, B_BLK, B_BLKOFF, B_NL, B_RSTART, B_RLEN, B_KC, B_KOFS, B_SMALL, B_BCNT, B_BOFF, B_BCUR,
   B_S, B_C, B_TU, B_UOFF, B_TK, B_TC, B_HIST, B_KBS, B_KBOFF, B_KK, B_KC2, B_MNQ, B_MNPLAN,
   B_MNNT, B_MNNG, B_MNNOFF, B_MNFLAG, B_NODES, B_TMP, B_OVF, B_PT0, B_PT1, B_PT2, B_PT3, B_PT4,
-  B_PT5, B_PT6, B_BLKGT, B_S2, B_BCNT2, B_BOFF2, B_BCUR2, B_TP, B_HIST1, B_OFF1, B_HIST2, B_OFF2, B_CUTBH, B_LFREQ, B_WORDER, B_WIRE, B_NORM, B_NORMNL, B_NORMLEN, B_NORMOFF, B_NORMLS, B_NORMLL, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
+  B_PT5, B_PT6, B_BLKGT, B_S2, B_BCNT2, B_BOFF2, B_BCUR2, B_TP, B_HIST1, B_OFF1, B_HIST2, B_OFF2, B_CUTBH, B_LFREQ, B_WORDER, B_WIRE, B_NORM, B_NORMNL, B_NORMLEN, B_NORMOFF, B_NORMLS, B_NORMLL, B_SEGH, B_SEGO, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
 };
 
 }  // namespace
@@ -2445,6 +2661,7 @@ struct pk_ctx {
   // start/end, scatter start/end) for the first MS_TIMED_LEVELS levels
   hipEvent_t ev[20 + 4 * 4] = {};
   int ms_levels = 0;  // further partition levels of the last count
+  int ms_form[MS_TIMED_LEVELS] = {};  // their scatter forms (timed levels)
   pk_timing timing{};
   // state of the current table
   int k = 0;
@@ -2733,7 +2950,7 @@ int resolve_overflow(pk_ctx* c, Key* S, Key* R, int32_t* C, int K, int fixed_bit
 // Index of the event bytes ('\n', or '\n' and '>') of fa: *m events at
 // *ev_out.  small[0] receives the position of the first '>'.
 int event_index(pk_ctx* c, const uint8_t* fa, uint64_t n, uint8_t e0, uint8_t e1, ull* small,
-                ull** ev_out, uint64_t* m_out, uint64_t* p0_out, int ev_buf = B_NL) {
+                ull** ev_out, uint64_t* m_out, uint64_t* p0_out, int ev_buf = B_NL, int follow = 0) {
   hipStream_t st = c->stream;
   const uint64_t nb = (n + NL_PER_BLOCK - 1) / NL_PER_BLOCK;
   ull *blk, *blkoff, *nl, *blkgt;
@@ -2754,7 +2971,7 @@ int event_index(pk_ctx* c, const uint8_t* fa, uint64_t n, uint8_t e0, uint8_t e1
   const uint64_t m = hm[0];
   PK_TRY(need(c, ev_buf, m + 1, &nl));
   if (nb) {
-    hipLaunchKernelGGL(k_nl_write, dim3((unsigned)nb), dim3(NL_NT), 0, st, fa, n, e0, e1, blkoff, nl);
+    hipLaunchKernelGGL(k_nl_write, dim3((unsigned)nb), dim3(NL_NT), 0, st, fa, n, e0, e1, blkoff, nl, follow);
     PK_HIP(c, hipGetLastError());
   }
   *ev_out = nl;
@@ -2836,9 +3053,11 @@ int records_main(pk_ctx* c, const uint8_t* fa, uint64_t n, int K, ull** rs, uint
   // [3] = first over-long read, [4] = overflow count
   PK_TRY(fill_u64(c, small, ~0ull, 4));
   PK_TRY(fill_u64(c, small + 4, 0, 1));
+  if (n >> NL_FLAG_SHIFT) return c->fail(PK_EINVAL, "input larger than 2^62 bytes");
   ull* nl;
   uint64_t m, p0;
-  PK_TRY(event_index(c, fa, n, '\n', '\n', small, &nl, &m, &p0));
+  // entries carry what follows each '\n' (bits 62-63): only k_reads reads them
+  PK_TRY(event_index(c, fa, n, '\n', '\n', small, &nl, &m, &p0, B_NL, 1));
   const uint64_t ncand = m / 2 + 2;
   ull *rstart, *kofs;
   uint32_t *rlen, *kc;
@@ -2846,7 +3065,7 @@ int records_main(pk_ctx* c, const uint8_t* fa, uint64_t n, int K, ull** rs, uint
   PK_TRY(need(c, B_RLEN, ncand, &rlen));
   PK_TRY(need(c, B_KC, ncand, &kc));
   PK_TRY(need(c, B_KOFS, ncand + 1, &kofs));
-  hipLaunchKernelGGL(k_reads, dim3(grid_for(ncand, 256)), dim3(256), 0, st, fa, n, nl, m, p0, K,
+  hipLaunchKernelGGL(k_reads, dim3(grid_for(ncand, 256)), dim3(256), 0, st, n, nl, m, p0, K,
                      ncand, rstart, rlen, kc, small + 1, small + 2, small + 3);
   PK_HIP(c, hipGetLastError());
   PK_TRY(scan_excl<uint32_t>(c, kc, kofs, ncand));
@@ -2912,17 +3131,20 @@ int records_builder(pk_ctx* c, const uint8_t* fa, uint64_t n, int klen, uint64_t
 // table.  S holds the N keys (KB key bits) split into B buckets by their top
 // `fixed` bits, at offsets boff (the B_BOFF buffer); levels of <= MS_MAX_BITS
 // follow until btot bits are fixed.  S2 is the other instance array, C the
-// per-instance counts; small[4] (the overflow count) starts at zero.
+// per-instance counts; small[4] (the overflow count) starts at zero.  set0 =
+// 1 when boff is in B_BOFF2.
 template <class Key>
 int count_partitioned(pk_ctx* c, Key* S, Key* S2, int32_t* C, ull* boff, uint64_t B, int fixed, int btot,
-                      int KB, uint64_t N) {
+                      int KB, uint64_t N, int set0 = 0) {
   Key* TK;
   int32_t* TC;
   hipStream_t st = c->stream;
   const pk_tuning& tun = c->tuning;
   ull* small;
   PK_TRY(need(c, B_SMALL, 16, &small));
-  int set = 0;
+  // set: which offset / histogram buffers the input level used (boff sits
+  // in B_BOFF when 0, B_BOFF2 when 1)
+  int set = set0;
   Key* Sin = S;
   Key* Sout = S2;
   constexpr int TQ = MS_NT * TileCfg<Key>::Q;
@@ -2968,6 +3190,7 @@ int count_partitioned(pk_ctx* c, Key* S, Key* S2, int32_t* C, ull* boff, uint64_
     const bool persistent = tun.ms_scatter ? tun.ms_scatter == 1 : ntiles >= 256 * pgrid;
     if (lv >= 0) PK_HIP(c, hipEventRecord(c->ev[22 + 4 * lv], st));
     c->timing.ms_scatter_form = persistent ? 1 : 2;
+    if (lv >= 0) c->ms_form[lv] = persistent ? 1 : 2;
     if (persistent)
       hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_scatter_p<Key, MS_P_NT>), dim3((unsigned)pgrid), dim3(MS_P_NT), 0, st,
                          Sin, boff, tp, (int)B, dpos, db, O2, Sout);
@@ -3155,9 +3378,32 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   PK_TRY(need(c, B_S2, N, &S2));
   PK_TRY(need(c, B_C, N, &C));
   c->timing.bucket_bits = btot;
+  // fold the second level's histogram into this one when two or more levels
+  // follow level 1 and its (bucket, digit) counters fit 32 KB of LDS
+  constexpr int sgf = L1Scatter<Key, ENC, OWN>::GF;
+  L2Fold f2{0, sgf, 0, nullptr};
+  {
+    const int left = btot - d1;
+    const int nlev = (left + MS_MAX_BITS - 1) / MS_MAX_BITS;
+    const int db = nlev > 0 ? (left + nlev - 1) / nlev : 0;
+    if (L1Scatter<Key, ENC, OWN>::main && nlev >= 2 && (B1 << db) <= L2_FOLD_MAX && tun.ms_fold != 2) {
+      f2.d2 = db;
+      // segments of about four level-2 tiles on average (whole scatter
+      // blocks): a segment's last tile is a partial one
+      constexpr uint64_t TQ2 = (uint64_t)MS_NT * TileCfg<Key>::Q;
+      const uint64_t want = (nblk * B1 * 4 * TQ2 + N - 1) / N;
+      const uint64_t g = std::max<uint64_t>(1, (want + sgf - 1) / sgf);
+      f2.spb = (int)std::min<uint64_t>(g * sgf, (nblk + sgf - 1) / sgf * sgf);
+      f2.nsb = (nblk + f2.spb - 1) / f2.spb;
+      const uint64_t nh2 = (B1 << db) * f2.nsb;
+      PK_TRY(need(c, B_SEGH, nh2 + 2, &f2.H2));
+      PK_TRY(fill_u64(c, reinterpret_cast<ull*>(f2.H2), 0, (nh2 + 1) / 2));
+    }
+  }
   PK_HIP(c, hipEventRecord(c->ev[2], st));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_hist<Key, ENC, OWN>), dim3((unsigned)nblk), dim3(L1_NT),
-                     l1_lds_bytes(sb, (int)B1, 4), st, fa, n, rstart, rlen, R, K, KB, mask, d1, rpb, sb, cf, H);
+                     l1_lds_bytes(sb, (int)B1, 4) + (f2.d2 ? ((uint32_t)B1 << f2.d2) * 4u : 0u), st, fa, n, rstart,
+                     rlen, R, K, KB, mask, d1, rpb, sb, cf, H, f2);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[3], st));
   PK_TRY(scan_excl<uint32_t>(c, H, O, B1 * nblk));
@@ -3180,13 +3426,33 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
                      boff);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[4], st));
-  constexpr int sgf = L1Scatter<Key, ENC, OWN>::GF;
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_l1_scatter<Key, ENC, OWN>), dim3((unsigned)((nblk + sgf - 1) / sgf)),
                      dim3(snt), l1_lds_bytes(sbs, (int)B1, 12), st, fa, n, rstart, rlen, R, K, KB, mask, d1, rpb,
                      sbs, cf, O, nblk, S);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[16], st));
-  return count_partitioned<Key>(c, S, S2, C, boff, B1, d1, btot, KB, N);
+  if (!f2.d2) return count_partitioned<Key>(c, S, S2, C, boff, B1, d1, btot, KB, N);
+  // level 2 from the folded counts: scan them in (bucket, digit, segment)
+  // order, then the segment scatter
+  const int db = f2.d2;
+  const uint64_t B2 = B1 << db, nh2 = B2 * f2.nsb;
+  ull *O2, *boff2;
+  PK_TRY(need(c, B_SEGO, nh2 + 1, &O2));
+  PK_TRY(need(c, B_BOFF2, B2 + 1, &boff2));
+  PK_HIP(c, hipEventRecord(c->ev[20], st));
+  PK_TRY(scan_excl<uint32_t>(c, f2.H2, O2, nh2));
+  hipLaunchKernelGGL(k_seg_boff, dim3(grid_for(B2 + 1, 256)), dim3(256), 0, st, O2, f2.nsb, (ull)B2, (ull)N, boff2);
+  PK_HIP(c, hipGetLastError());
+  PK_HIP(c, hipEventRecord(c->ev[21], st));
+  PK_HIP(c, hipEventRecord(c->ev[22], st));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_scatter_seg<Key, MS_P_NT>), dim3(2u * (unsigned)c->ncu), dim3(MS_P_NT), 0, st,
+                     S, O, nblk, f2.spb, f2.nsb, (int)B1, (ull)N, KB - d1 - db, db, O2, S2);
+  PK_HIP(c, hipGetLastError());
+  PK_HIP(c, hipEventRecord(c->ev[23], st));
+  c->timing.ms_scatter_form = 3;
+  c->ms_form[0] = 3;
+  c->ms_levels = 1;
+  return count_partitioned<Key>(c, S2, S, C, boff2, B2, d1 + db, btot, KB, N, 1);
 }
 
 // Distinct klen-byte windows of the implicit builder counters: keys from
@@ -3686,7 +3952,8 @@ int pk_ctx_set_tuning(pk_ctx* c, const pk_tuning* t) {
     return PK_OK;
   }
   if (t->l1_chunks < 0 || t->l1_chunks > 64 || t->l1_bits < 0 || t->l1_bits > L1_MAX_BITS ||
-      !(t->bucket_factor >= 0.f && t->bucket_factor <= 64.f) || t->ms_scatter < 0 || t->ms_scatter > 2)
+      !(t->bucket_factor >= 0.f && t->bucket_factor <= 64.f) || t->ms_scatter < 0 || t->ms_scatter > 2 ||
+      t->ms_fold < 0 || t->ms_fold > 2)
     return c->fail(PK_EINVAL, "tuning values out of range");
   c->tuning = *t;
   return PK_OK;
@@ -4236,9 +4503,13 @@ int pk_last_timing(const pk_ctx* cc, pk_timing* out) {
     c->timing.compact_ms = el(10, 11);
     const int nl = c->ms_levels < MS_TIMED_LEVELS ? c->ms_levels : MS_TIMED_LEVELS;
     c->timing.ms_hist_ms = c->timing.ms_scatter_ms = 0.f;
-    for (int l = 0; l < nl; l++) {
-      c->timing.ms_hist_ms += el(20 + 4 * l, 21 + 4 * l);
-      c->timing.ms_scatter_ms += el(22 + 4 * l, 23 + 4 * l);
+    for (int l = 0; l < MS_TIMED_LEVELS; l++) {
+      const bool on = l < nl;
+      c->timing.ms_level_hist_ms[l] = on ? el(20 + 4 * l, 21 + 4 * l) : 0.f;
+      c->timing.ms_level_scatter_ms[l] = on ? el(22 + 4 * l, 23 + 4 * l) : 0.f;
+      c->timing.ms_level_form[l] = on ? c->ms_form[l] : 0;
+      c->timing.ms_hist_ms += c->timing.ms_level_hist_ms[l];
+      c->timing.ms_scatter_ms += c->timing.ms_level_scatter_ms[l];
     }
     c->timing.ms_levels = c->ms_levels;
     c->pend_count = false;

@@ -441,7 +441,7 @@ int pk_server_set_tuning(pk_server* s, const pk_tune_entry* entries, int n) {
     const pk_tuning& t = e.tuning;
     if (e.key_words < 0 || e.key_words > 2 || t.l1_chunks < 0 || t.l1_chunks > 64 || t.l1_bits < 0 ||
         t.l1_bits > PK_TUNING_L1_BITS_MAX || !(t.bucket_factor >= 0.f && t.bucket_factor <= 64.f) ||
-        t.ms_scatter < 0 || t.ms_scatter > 2)
+        t.ms_scatter < 0 || t.ms_scatter > 2 || t.ms_fold < 0 || t.ms_fold > 2)
       return s->fail(PK_EINVAL, "tuning entry " + std::to_string(i) + " out of range");
   }
   try {

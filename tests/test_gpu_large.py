@@ -12,8 +12,8 @@ and the macro nodes are checked through properties that hold at any size:
   * every kept k-mer is the suffix of exactly one node and the prefix of
     exactly one node (so the counts of either side sum to the kept counts),
     node keys strictly ascending;
-  * a second count with the other partition-scatter form gives the same
-    table (digest of keys and counts).
+  * a second count with the other partition-scatter form and without the
+    level-2 histogram fold gives the same table (digest of keys and counts).
 The same pipeline at 2 % of this size is compared with the oracle entry for
 entry in test_gpu_parity.py (test_repeat_family_k64)."""
 from __future__ import annotations
@@ -126,9 +126,9 @@ def test_large_full_size_properties(large):
     del nodes, kk, kc
     d0 = _digest(keys, counts)
     del keys, counts, lo, hi
-    _p(f"  {m} nodes; recount with one block per tile in the further partition levels")
+    _p(f"  {m} nodes; recount with one block per tile in the further partition levels, no fold")
     try:
-        ctx.set_tuning(ms_scatter=2)
+        ctx.set_tuning(ms_scatter=2, ms_fold=2)
         ctx.count(fa, k)
         keys2, counts2 = ctx.table_host()
     finally:

@@ -390,8 +390,12 @@ def test_macro_nodes_rank_share(ctx, oracle_mod, name, k, t, P):
                                     # both forms of the further levels' scatter
                                     dict(ms_scatter=1), dict(ms_scatter=2, bucket_factor=1.5),
                                     # tiny buckets: three partition levels with the
-                                    # built-in level-1 width
-                                    dict(bucket_factor=0.001)])
+                                    # built-in level-1 width; the second level's
+                                    # counts folded into the level-1 histogram
+                                    # (built-in) or counted in their own pass
+                                    dict(bucket_factor=0.001), dict(bucket_factor=0.001, ms_fold=2),
+                                    dict(bucket_factor=0.003, l1_bits=3),
+                                    dict(bucket_factor=0.001, ms_scatter=2)])
 def test_tuning_does_not_change_results(ctx, oracle_mod, tuning):
     """Launch-shape choices of the autotuner leave every output unchanged."""
     _p(f"case: tuning {tuning}")
