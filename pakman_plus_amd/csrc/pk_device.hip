@@ -1357,7 +1357,8 @@ __global__ __launch_bounds__(NT, NT >= 1024 ? 8 : 4) void k_ms_scatter_p(const K
 // Level 2 over the segments of the folded histogram (L2Fold): segment j =
 // (a, g), a = j / nsb, holds [O1[a * nblk + g * spb], O1[a * nblk +
 // min((g + 1) * spb, nblk)]) of the level-1 array; its run of sub-digit s
-// starts at O2[((a << dbits) + s) * nsb + g] (the scan of H2).  Block i takes
+// starts at O2T[(j << dbits) + s] (the scan of H2, transposed so that a
+// segment's bases are one contiguous run).  Block i takes
 // the segments that start in its 1/G share of the keys and sorts them tile by
 // tile (TQ keys) as k_ms_scatter_p does, the next tile's keys loaded ahead;
 // within a segment each tile's runs start where the previous tile's ended.
@@ -1384,7 +1385,7 @@ __device__ __forceinline__ ull seg_first_at(const ull* O1, uint64_t nblk, int sp
 template <class Key, int NT>
 __global__ __launch_bounds__(NT, 4) void k_ms_scatter_seg(const Key* Sin, const ull* O1, uint64_t nblk, int spb,
                                                           uint64_t nsb, int nb1, ull total, int dpos, int dbits,
-                                                          const ull* O2, Key* Sout) {
+                                                          const ull* O2T, Key* Sout) {
   constexpr int TQ = MS_NT * TileCfg<Key>::Q;
   constexpr int Q = TQ / NT;
   __shared__ TileLds<Key, MS_NT> L;
@@ -1403,10 +1404,7 @@ __global__ __launch_bounds__(NT, 4) void k_ms_scatter_seg(const Key* Sin, const 
     j++;
   }
   if (j >= j1) return;
-  auto base_of = [&](ull jj) -> ull {
-    const ull a = jj / nsb, sb = jj - a * nsb;
-    return O2[((a << dbits) + (ull)tid) * nsb + sb];
-  };
+  auto base_of = [&](ull jj) -> ull { return O2T[(jj << dbits) + (ull)tid]; };
   ull pos = st;
   int m = (int)(en - pos < (ull)TQ ? en - pos : (ull)TQ);
   Key kr[Q];
@@ -1484,11 +1482,37 @@ __global__ __launch_bounds__(NT, 4) void k_ms_scatter_seg(const Key* Sin, const 
   }
 }
 
-// boff2[(a << dbits) | s] = O2[((a << dbits) | s) * nsb] (the first segment's
-// base), boff2[end] = total
-__global__ void k_seg_boff(const ull* O2, uint64_t nsb, ull n2, ull total, ull* boff2) {
+// O2T[((a * nsb + g) << db) + s] = O2[((a << db) + s) * nsb + g]: per
+// level-1 bucket a, the (digit, segment) matrix of bases transposed through
+// LDS in 32 x 32 tiles (grid: segment tiles, digit tiles, a).
+__global__ __launch_bounds__(256) void k_seg_transpose(const ull* O2, uint64_t nsb, int db, ull* O2T) {
+  __shared__ ull t[32][33];
+  const uint64_t a = blockIdx.z;
+  const int S = 1 << db;
+  const uint64_t g0 = (uint64_t)blockIdx.x * 32;
+  const int s0 = (int)blockIdx.y * 32;
+  const int tx = (int)threadIdx.x & 31, ty = (int)threadIdx.x >> 5;
+  const ull* src = O2 + ((a << db) * nsb);
+  for (int r = ty; r < 32; r += 8) {
+    const int sd = s0 + r;
+    const uint64_t g = g0 + (uint64_t)tx;
+    if (sd < S && g < nsb) t[r][tx] = src[(uint64_t)sd * nsb + g];
+  }
+  __syncthreads();
+  ull* dst = O2T + ((a * nsb) << db);
+  for (int r = ty; r < 32; r += 8) {
+    const uint64_t g = g0 + (uint64_t)r;
+    const int sd = s0 + tx;
+    if (sd < S && g < nsb) dst[(g << db) + (uint64_t)sd] = t[tx][r];
+  }
+}
+
+// boff2[(a << db) | s] = the first segment's base, O2T[((a * nsb) << db) + s];
+// boff2[end] = total
+__global__ void k_seg_boff(const ull* O2T, uint64_t nsb, int db, ull n2, ull total, ull* boff2) {
   const ull g = (ull)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g < n2) boff2[g] = O2[g * nsb];
+  const ull mask = (1ull << db) - 1;
+  if (g < n2) boff2[g] = O2T[(((g >> db) * nsb) << db) + (g & mask)];
   if (g == n2) boff2[n2] = total;
 }
 
@@ -1531,15 +1555,6 @@ constexpr int HASH_PROBES = 256;
 // flag every 8 probes: once one thread has found the table full, the others
 // stop probing it (short chains, the usual case, never read it)
 constexpr int STOP_AFTER = 8;
-
-__device__ __forceinline__ ull mix64(ull x) {
-  x ^= x >> 33;
-  x *= 0xff51afd7ed558ccdull;
-  x ^= x >> 33;
-  x *= 0xc4ceb9fe1a85ec53ull;
-  x ^= x >> 33;
-  return x;
-}
 
 template <class Key>
 struct HashTable;
@@ -1597,8 +1612,17 @@ struct HashTable<u128> {
   u128* hkf;
   uint32_t* hc;
   ull empty;
+  // the high word through one odd multiply (a bijection) folded into the
+  // low word, then one multiply-xorshift round for the top bits the home
+  // slot takes: two 64-bit products instead of the four of two full mixes
+  // (the table is bound by instruction issue).  Distinct keys of a task
+  // collide with probability ~2^-64 per pair; a collision only sends the
+  // task to the split path.
   __device__ __forceinline__ static ull fp(u128 k) {
-    return mix64((ull)k ^ mix64((ull)(k >> 64) + 0x632be59bd9b4e019ull));
+    ull f = (ull)k ^ ((ull)(k >> 64) * 0x9e3779b97f4a7c15ull);
+    f ^= f >> 29;
+    f *= 0xbf58476d1ce4e5b9ull;
+    return f ^ (f >> 32);
   }
   __device__ __forceinline__ void init(int tid, int nt) {
     for (int i = tid; i < T; i += nt) {
@@ -2644,7 +2668,7 @@ enum BufId {
   B_FASTA, B_BLK, B_BLKOFF, B_NL, B_RSTART, B_RLEN, B_KC, B_KOFS, B_SMALL, B_BCNT, B_BOFF, B_BCUR,
   B_S, B_C, B_TU, B_UOFF, B_TK, B_TC, B_HIST, B_KBS, B_KBOFF, B_KK, B_KC2, B_MNQ, B_MNPLAN,
   B_MNNT, B_MNNG, B_MNNOFF, B_MNFLAG, B_NODES, B_TMP, B_OVF, B_PT0, B_PT1, B_PT2, B_PT3, B_PT4,
-  B_PT5, B_PT6, B_BLKGT, B_S2, B_BCNT2, B_BOFF2, B_BCUR2, B_TP, B_HIST1, B_OFF1, B_HIST2, B_OFF2, B_CUTBH, B_LFREQ, B_WORDER, B_WIRE, B_NORM, B_NORMNL, B_NORMLEN, B_NORMOFF, B_NORMLS, B_NORMLL, B_SEGH, B_SEGO, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
+  B_PT5, B_PT6, B_BLKGT, B_S2, B_BCNT2, B_BOFF2, B_BCUR2, B_TP, B_HIST1, B_OFF1, B_HIST2, B_OFF2, B_CUTBH, B_LFREQ, B_WORDER, B_WIRE, B_NORM, B_NORMNL, B_NORMLEN, B_NORMOFF, B_NORMLS, B_NORMLL, B_SEGH, B_SEGO, B_SEGT, B_SCAN0, B_SCAN1 = B_SCAN0 + 8, B_COUNT = B_SCAN0 + 16
 };
 
 }  // namespace
@@ -3436,17 +3460,22 @@ int count_records(pk_ctx* c, const uint8_t* fa, uint64_t n, const ull* rstart, c
   // order, then the segment scatter
   const int db = f2.d2;
   const uint64_t B2 = B1 << db, nh2 = B2 * f2.nsb;
-  ull *O2, *boff2;
+  ull *O2, *O2T, *boff2;
   PK_TRY(need(c, B_SEGO, nh2 + 1, &O2));
+  PK_TRY(need(c, B_SEGT, nh2, &O2T));
   PK_TRY(need(c, B_BOFF2, B2 + 1, &boff2));
   PK_HIP(c, hipEventRecord(c->ev[20], st));
   PK_TRY(scan_excl<uint32_t>(c, f2.H2, O2, nh2));
-  hipLaunchKernelGGL(k_seg_boff, dim3(grid_for(B2 + 1, 256)), dim3(256), 0, st, O2, f2.nsb, (ull)B2, (ull)N, boff2);
+  hipLaunchKernelGGL(k_seg_transpose, dim3((unsigned)((f2.nsb + 31) / 32), (unsigned)(((1u << db) + 31) / 32), (unsigned)B1),
+                     dim3(256), 0, st, O2, f2.nsb, db, O2T);
+  PK_HIP(c, hipGetLastError());
+  hipLaunchKernelGGL(k_seg_boff, dim3(grid_for(B2 + 1, 256)), dim3(256), 0, st, O2T, f2.nsb, db, (ull)B2, (ull)N,
+                     boff2);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[21], st));
   PK_HIP(c, hipEventRecord(c->ev[22], st));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ms_scatter_seg<Key, MS_P_NT>), dim3(2u * (unsigned)c->ncu), dim3(MS_P_NT), 0, st,
-                     S, O, nblk, f2.spb, f2.nsb, (int)B1, (ull)N, KB - d1 - db, db, O2, S2);
+                     S, O, nblk, f2.spb, f2.nsb, (int)B1, (ull)N, KB - d1 - db, db, O2T, S2);
   PK_HIP(c, hipGetLastError());
   PK_HIP(c, hipEventRecord(c->ev[23], st));
   c->timing.ms_scatter_form = 3;
