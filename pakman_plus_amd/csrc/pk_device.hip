@@ -81,6 +81,22 @@ __device__ __forceinline__ uint32_t key_digit(Key x, int pos, int nbits) {
   return (uint32_t)y & ((1u << nbits) - 1u);
 }
 
+// u128 keys: the digit from a 32-bit funnel shift of the two 32-bit words
+// around pos (v_alignbit), the words picked by a uniform pos without
+// branches; same value as the generic form for nbits <= 32.
+template <>
+__device__ __forceinline__ uint32_t key_digit<unsigned __int128>(unsigned __int128 x, int pos, int nbits) {
+  if (nbits <= 0) return 0;
+  if (pos < 0) return (uint32_t)(x << (-pos)) & ((1u << nbits) - 1u);
+  if (pos >= 128) return 0;
+  const uint64_t lo = (uint64_t)x, hi = (uint64_t)(x >> 64);
+  const uint32_t w0 = (uint32_t)lo, w1 = (uint32_t)(lo >> 32), w2 = (uint32_t)hi, w3 = (uint32_t)(hi >> 32);
+  const int wi = pos >> 5;
+  const uint32_t a = wi == 0 ? w0 : wi == 1 ? w1 : wi == 2 ? w2 : w3;
+  const uint32_t b = wi == 0 ? w1 : wi == 1 ? w2 : wi == 2 ? w3 : 0u;
+  return __builtin_amdgcn_alignbit(b, a, (uint32_t)(pos & 31)) & ((1u << nbits) - 1u);
+}
+
 // ---------------------------------------------------------------------------
 // Block-wide primitives (wave64).
 
