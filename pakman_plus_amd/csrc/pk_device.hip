@@ -1015,6 +1015,46 @@ __device__ __forceinline__ int walk_read(const ChunkLds& C, Stage sg, const uint
   return q;
 }
 
+// The histogram's digit of a main-path k-mer is its top d <= 13 key bits:
+// bases 0..6 of the window (key slot j holds base j's 2-bit code at bits
+// 2(K-1-j) and 2(K-1-j)+1, and the third code bit of base j+1 ORs into the
+// low bit of slot j).  A 15-bit register rolled like the key, ((v << 2) |
+// code) & 0x7FFF, holds slots 0..6 of the window that starts 6 bases back
+// (bit 14, the spill of the bases before, is never read; bit 0 still lacks
+// the next base's third bit, so d stays <= 13), and digit = (v >> (14 - d))
+// & (2^d - 1).  The walk stops at the last window's
+// seventh base: len - K + 7 bases per read instead of len, no K-bit key.
+// For K >= 7 only.
+template <class Emit>
+__device__ __forceinline__ void walk_read_top(const ChunkLds& C, Stage sg, const uint8_t* fa, ull s, uint32_t len,
+                                              int K, Emit&& emit) {
+  if (len < (uint32_t)K) return;
+  const uint32_t end = len - (uint32_t)K + 7;  // bases to read
+  uint32_t v = 0, i = 0;
+  if (C.lds) {
+    while (i < end) {
+      const ull off = s + i - C.base;
+      const uint32_t sh = (uint32_t)(off & 31);
+      ull a = sg.A[off >> 5] >> (2 * sh);
+      uint32_t b = sg.B[off >> 5] >> sh;
+      int nb = 32 - (int)sh;
+      while (nb > 0 && i < end) {
+        v = ((v << 2) | ((uint32_t)a & 3u) | ((b & 1u) << 2)) & 0x7FFFu;
+        a >>= 2;
+        b >>= 1;
+        nb--;
+        if (i >= 6) emit(v);
+        i++;
+      }
+    }
+    return;
+  }
+  for (; i < end; i++) {
+    v = ((v << 2) | base_code<ENC_MAIN>(fa[s + i])) & 0x7FFFu;
+    if (i >= 6) emit(v);
+  }
+}
+
 // L-mer frequencies (Sliding_window_l, kmer_counting.cpp:391-438): every
 // L-mer of every record's read line, rolling 3-bit update masked to 2L bits.
 __global__ void k_lmer_freq(const uint8_t* fa, const ull* rstart, const uint32_t* rlen, uint64_t R,
@@ -1054,21 +1094,33 @@ __global__ __launch_bounds__(L1_NT) void k_l1_hist(const uint8_t* fa, uint64_t n
   const uint64_t r0 = (uint64_t)blockIdx.x * rpb;
   const uint64_t r1 = r0 + rpb < R ? r0 + rpb : R;
   const int bpos = KB - d1;
+  // the main path's digits (<= 13 bits: the register's bit 0, the low bit of
+  // slot 6, still lacks the next base's third code bit) come from a 15-bit
+  // window of bases
+  const bool top = ENC == ENC_MAIN && !OWN && K >= 7 && KB == 2 * K && d1 + f2.d2 <= 13;
+  const int tsh = 14 - d1, tsh2 = 14 - d1 - f2.d2;
+  const uint32_t tm = (1u << d1) - 1u, tm2 = (1u << (d1 + f2.d2)) - 1u;
   for (uint64_t c0 = r0; c0 < r1;) {
     stage_chunk<ENC, L1_NT>(C, sg, fa, n, rstart, rlen, c0, r1);
     const int m = C.m;
     if ((int)threadIdx.x < m) {
       const uint64_t r = c0 + threadIdx.x;
-      uint32_t i = 0;
-      Key km = 0;
-      OwnState os{0, 0, 0, 0};
-      walk_read<Key, ENC, OWN>(C, sg, fa, rstart[r], rlen[r], i, km, os, cf, K, mask, 0x7fffffff,
-                               [&](int, Key x) {
-                                 // one LDS atomic per key: with the fold the
-                                 // level-1 counts are sums of h2's rows
-                                 if (nb2) atomicAdd(&h2[key_digit<Key>(x, bpos - f2.d2, d1 + f2.d2)], 1u);
-                                 else atomicAdd(&h[key_digit<Key>(x, bpos, d1)], 1u);
-                               });
+      if (top) {
+        // main path: the digit from the window's first seven bases
+        if (nb2) walk_read_top(C, sg, fa, rstart[r], rlen[r], K, [&](uint32_t v) { atomicAdd(&h2[(v >> tsh2) & tm2], 1u); });
+        else walk_read_top(C, sg, fa, rstart[r], rlen[r], K, [&](uint32_t v) { atomicAdd(&h[(v >> tsh) & tm], 1u); });
+      } else {
+        uint32_t i = 0;
+        Key km = 0;
+        OwnState os{0, 0, 0, 0};
+        walk_read<Key, ENC, OWN>(C, sg, fa, rstart[r], rlen[r], i, km, os, cf, K, mask, 0x7fffffff,
+                                 [&](int, Key x) {
+                                   // one LDS atomic per key: with the fold the
+                                   // level-1 counts are sums of h2's rows
+                                   if (nb2) atomicAdd(&h2[key_digit<Key>(x, bpos - f2.d2, d1 + f2.d2)], 1u);
+                                   else atomicAdd(&h[key_digit<Key>(x, bpos, d1)], 1u);
+                                 });
+      }
     }
     c0 += m;
     __syncthreads();
