@@ -978,6 +978,9 @@ __device__ __forceinline__ bool own_step(OwnState& o, const OwnCfg& cf, Key km, 
   return lmer_owner(o.min_lmer, cf.P) == cf.rank;
 }
 
+// km rolls unmasked (bits above the key shift out of the word, and the
+// key's own bits are the same either way): the mask is applied to the keys
+// handed out, once per k-mer instead of once per base.
 template <class Key, int ENC, bool OWN, class Emit>
 __device__ __forceinline__ int walk_read(const ChunkLds& C, Stage sg, const uint8_t* fa, ull s, uint32_t len,
                                          uint32_t& i,
@@ -994,23 +997,23 @@ __device__ __forceinline__ int walk_read(const ChunkLds& C, Stage sg, const uint
       int nb = 32 - (int)sh;
       while (nb > 0 && q < Q && i < len) {
         const uint32_t el = ((uint32_t)a & 3u) | ((b & 1u) << 2);
-        km = ((km << 2) | (Key)el) & mask;
-        const bool own = own_step<Key, OWN>(os, cf, km, el, i, K);
+        km = (km << 2) | (Key)el;
+        const bool own = own_step<Key, OWN>(os, cf, OWN ? km & mask : km, el, i, K);
         a >>= 2;
         b >>= 1;
         nb--;
         i++;
-        if (i >= (uint32_t)K && own) emit(q++, km);
+        if (i >= (uint32_t)K && own) emit(q++, km & mask);
       }
     }
     return q;
   }
   while (q < Q && i < len) {
     const uint32_t el = base_code<ENC>(fa[s + i]);
-    km = ((km << 2) | (Key)el) & mask;
-    const bool own = own_step<Key, OWN>(os, cf, km, el, i, K);
+    km = (km << 2) | (Key)el;
+    const bool own = own_step<Key, OWN>(os, cf, OWN ? km & mask : km, el, i, K);
     i++;
-    if (i >= (uint32_t)K && own) emit(q++, km);
+    if (i >= (uint32_t)K && own) emit(q++, km & mask);
   }
   return q;
 }
