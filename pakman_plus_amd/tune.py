@@ -21,8 +21,9 @@ import time
 DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 # bucket upper bounds (FASTA bytes).  The samples stop near 1 GiB, so larger
 # inputs keep the built-in choices, which follow the bucket count (at 5.4 GB,
-# k = 64, the built-in 5-bit level 1 beats the 1 GiB sample's 10 bits,
-# 144 against 157 ms per step).
+# k = 64, three partition levels: the built-in 5-bit level 1 beat the 1 GiB
+# sample's 10 bits).  A bucket where no candidate beats the built-in choices
+# keeps them (zero fields).
 BUCKETS = [1 << 21, 1 << 23, 1 << 25, 1 << 27, 1 << 29, 1 << 31]
 CANDIDATES = ([{"l1_chunks": c, "l1_bits": 8, "bucket_factor": f}
                for c in (1, 2, 4, 8) for f in (1.5, 2.2, 3.0, 5.0)]
@@ -129,6 +130,9 @@ def _tune_k(reps: int, k: int, t: int) -> list[dict]:
             if r:
                 ts.append(time.perf_counter() - t0)
         e["builtin_ms"] = float(np.median(ts)) * 1e3
+        if e["builtin_ms"] < e["ms"]:
+            # no candidate beats the built-in choices: zero fields keep them
+            e.update(l1_chunks=0, l1_bits=0, bucket_factor=0.0, ms=e["builtin_ms"], builtin=True)
         entries.append(e)
         print(json.dumps(e), flush=True)
         prev = mb
