@@ -1738,6 +1738,26 @@ struct HashTable<u128> {
     return -1;
   }
   __device__ __forceinline__ bool insert(u128 k, const volatile int* stop) { return insert_at(k, stop) >= 0; }
+  // the first probe of insert_at alone: the home slot if it holds f or is
+  // claimed for it, else -1 (the slot holds another fingerprint)
+  __device__ __forceinline__ int try_home(u128 k, ull f) {
+    const uint32_t slot = home(f);
+    return try_slot(slot, f, k) ? (int)slot : -1;
+  }
+  // the rest of insert_at, for a key whose home slot holds another fingerprint
+  __device__ __forceinline__ int insert_after_home(u128 k, ull f, const volatile int* stop) {
+    uint32_t slot = next(home(f));
+    for (int probe = 1; probe < STOP_AFTER; probe++) {
+      if (try_slot(slot, f, k)) return (int)slot;
+      slot = next(slot);
+    }
+    for (int probe = STOP_AFTER; probe < HASH_PROBES; probe++) {
+      if ((probe & 7) == 0 && *stop) return -1;
+      if (try_slot(slot, f, k)) return (int)slot;
+      slot = next(slot);
+    }
+    return -1;
+  }
   // after a barrier: does key k sit in its fingerprint's slot?
   __device__ __forceinline__ bool verify(u128 k) const {
     const ull f = fp(k);
@@ -1804,6 +1824,7 @@ __global__ __launch_bounds__(BS_NT, HashCfg<Key>::BLOCKS) void k_bucket_hash(con
   __shared__ HashLds<Key> lds;
   __shared__ uint32_t hs[SUB_BINS];
   __shared__ int over;
+  __shared__ int qn;  // u128: keys of the batch queued for their probe chains
 
   const uint64_t task = blockIdx.x;
   if (task >= ntasks) return;
@@ -1837,23 +1858,68 @@ __global__ __launch_bounds__(BS_NT, HashCfg<Key>::BLOCKS) void k_bucket_hash(con
       // batch keeps them there for the fingerprint check below
       Key kb[HB];
       int ksl[HB];  // u128: the slot each key of the (last) batch landed in
+      int ov = 0;   // u128: over as read after the last batch's final barrier
       for (uint64_t b = 0; b < n; b += (uint64_t)BS_NT * HB) {
 #pragma unroll
         for (int j = 0; j < HB; j++) {
           const uint64_t idx = b + threadIdx.x + (uint64_t)j * BS_NT;
           kb[j] = idx < n ? S[start + idx] : Key(0);
         }
-        if (*(volatile int*)&over) break;
+        if constexpr (sizeof(Key) == 16) {
+          // A wave runs each probe step as long as its slowest lane needs
+          // it, so a batch goes in two phases: every key tries its home slot
+          // (one probe, the common case), and the keys whose home slot holds
+          // another fingerprint queue up (in hs, idle until the output
+          // phase) and are then probed by all lanes together, so a long
+          // chain holds up one queue round instead of each of the eight
+          // keys of its lane.  Every thread takes the overflow exit on the
+          // value read after the previous batch's last barrier (nothing
+          // writes over between that read and the next barrier), so all
+          // leave the loop together.
+          if (ov) break;
+          if (threadIdx.x == 0) qn = 0;
+          __syncthreads();
+          int qp[HB];
 #pragma unroll
-        for (int j = 0; j < HB; j++) {
-          const uint64_t idx = b + threadIdx.x + (uint64_t)j * BS_NT;
-          if (idx < n && (int)key_digit<Key>(kb[j], rem - s, s) == pass) {
-            if constexpr (sizeof(Key) == 16) {
-              ksl[j] = ht.insert_at(kb[j], &over);
-              if (ksl[j] < 0) over = 1;
-            } else if (!ht.insert(kb[j], &over)) {
-              over = 1;
+          for (int j = 0; j < HB; j++) {
+            qp[j] = -1;
+            const uint64_t idx = b + threadIdx.x + (uint64_t)j * BS_NT;
+            if (idx < n && (int)key_digit<Key>(kb[j], rem - s, s) == pass) {
+              const ull f = HashTable<u128>::fp(kb[j]);
+              ksl[j] = f == HashTable<u128>::EMPTY ? -2 : ht.try_home(kb[j], f);
+              if (ksl[j] == -2) {
+                over = 1;
+              } else if (ksl[j] < 0) {
+                const int q = atomicAdd(&qn, 1);
+                if (q < SUB_BINS) {
+                  hs[q] = (uint32_t)(threadIdx.x + j * BS_NT);
+                  qp[j] = q;
+                } else {
+                  ksl[j] = ht.insert_after_home(kb[j], f, &over);
+                  if (ksl[j] < 0) over = 1;
+                }
+              }
             }
+          }
+          __syncthreads();
+          const int nq = qn < SUB_BINS ? qn : SUB_BINS;
+          for (int q = threadIdx.x; q < nq; q += BS_NT) {
+            const u128 k = S[start + b + hs[q]];
+            const int sl = ht.insert_after_home(k, HashTable<u128>::fp(k), &over);
+            if (sl < 0) over = 1;
+            hs[q] = (uint32_t)sl;
+          }
+          __syncthreads();
+          ov = over;
+#pragma unroll
+          for (int j = 0; j < HB; j++)
+            if (qp[j] >= 0) ksl[j] = (int)hs[qp[j]];
+        } else {
+          if (*(volatile int*)&over) break;
+#pragma unroll
+          for (int j = 0; j < HB; j++) {
+            const uint64_t idx = b + threadIdx.x + (uint64_t)j * BS_NT;
+            if (idx < n && (int)key_digit<Key>(kb[j], rem - s, s) == pass && !ht.insert(kb[j], &over)) over = 1;
           }
         }
       }
