@@ -2240,6 +2240,9 @@ template <class Key>
 __global__ __launch_bounds__(CUT_NT) void k_kept_write(const Key* TK, const int32_t* TC, uint64_t U,
                                                       const int* mb, const ull* boffs, Key* KK,
                                                       int32_t* KC) {
+  // a block with no kept entry reads nothing (high cutoffs keep a few
+  // thousand of a billion k-mers); the test is the same for every thread
+  if (boffs[blockIdx.x + 1] == boffs[blockIdx.x]) return;
   const int thr = *mb;
   const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER + (uint64_t)threadIdx.x * 4;
   int32_t cv[4];
