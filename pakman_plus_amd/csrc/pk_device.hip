@@ -1035,19 +1035,26 @@ __device__ __forceinline__ void walk_read_top(const ChunkLds& C, Stage sg, const
   const uint32_t end = len - (uint32_t)K + 7;  // bases to read
   uint32_t v = 0, i = 0;
   if (C.lds) {
+    // 32 bases at a time from the read's own position (two words shifted
+    // together), so every lane runs the same count but on its last piece
+    // (pieces ending at word boundaries differ per read and diverge)
     while (i < end) {
       const ull off = s + i - C.base;
+      const ull w = off >> 5;
       const uint32_t sh = (uint32_t)(off & 31);
-      ull a = sg.A[off >> 5] >> (2 * sh);
-      uint32_t b = sg.B[off >> 5] >> sh;
-      int nb = 32 - (int)sh;
-      while (nb > 0 && i < end) {
+      const uint32_t cnt = end - i < 32u ? end - i : 32u;
+      ull a = sg.A[w] >> (2 * sh);
+      uint32_t b = sg.B[w] >> sh;
+      if (sh + cnt > 32u) {
+        // the piece runs into the next word, which holds bases of this read
+        a |= sg.A[w + 1] << (64 - 2 * sh);
+        b |= sg.B[w + 1] << (32 - sh);
+      }
+      for (uint32_t q = 0; q < cnt; q++, i++) {
         v = ((v << 2) | ((uint32_t)a & 3u) | ((b & 1u) << 2)) & 0x7FFFu;
         a >>= 2;
         b >>= 1;
-        nb--;
         if (i >= 6) emit(v);
-        i++;
       }
     }
     return;
