@@ -988,20 +988,27 @@ __device__ __forceinline__ int walk_read(const ChunkLds& C, Stage sg, const uint
                                          Emit&& emit) {
   int q = 0;
   if (C.lds) {
-    // 32 staged codes per pair of LDS loads
+    // 32 staged codes at a time from the read's own position (two words
+    // shifted together), so every lane runs the same count but on its last
+    // piece (pieces ending at word boundaries differ per read and diverge)
     while (q < Q && i < len) {
       const ull off = s + i - C.base;
+      const ull w = off >> 5;
       const uint32_t sh = (uint32_t)(off & 31);
-      ull a = sg.A[off >> 5] >> (2 * sh);
-      uint32_t b = ENC == ENC_MAIN ? sg.B[off >> 5] >> sh : 0u;
-      int nb = 32 - (int)sh;
-      while (nb > 0 && q < Q && i < len) {
+      const uint32_t cnt = len - i < 32u ? len - i : 32u;
+      ull a = sg.A[w] >> (2 * sh);
+      uint32_t b = ENC == ENC_MAIN ? sg.B[w] >> sh : 0u;
+      if (sh + cnt > 32u) {
+        // the piece runs into the next word, which holds bases of this read
+        a |= sg.A[w + 1] << (64 - 2 * sh);
+        if (ENC == ENC_MAIN) b |= sg.B[w + 1] << (32 - sh);
+      }
+      for (uint32_t t = 0; t < cnt && q < Q; t++) {
         const uint32_t el = ((uint32_t)a & 3u) | ((b & 1u) << 2);
         km = (km << 2) | (Key)el;
         const bool own = own_step<Key, OWN>(os, cf, OWN ? km & mask : km, el, i, K);
         a >>= 2;
         b >>= 1;
-        nb--;
         i++;
         if (i >= (uint32_t)K && own) emit(q++, km & mask);
       }
