@@ -2031,8 +2031,14 @@ __global__ __launch_bounds__(BS_NT, HashCfg<Key>::BLOCKS) void k_bucket_hash(con
         // array, so that six tables fit a CU's LDS)
         const int a = (int)hs[d];
         const int e = d + 1 < nbins ? (int)hs[d + 1] : (int)nd;
-        int rank = 0;
-        for (int j = a; j < e; j++) rank += ok[j] < v;
+        // the sub-bin's first four slots read at once (most hold one or two
+        // keys), the rest of a longer sub-bin one by one.  a < nd <= T, so a
+        // read past the key array lands in the counts that follow it in the
+        // same LDS struct, and only slots below e count.
+        const Key w0 = ok[a], w1 = ok[a + 1], w2 = ok[a + 2], w3 = ok[a + 3];
+        int rank = (int)(w0 < v) + (int)(a + 1 < e && w1 < v) + (int)(a + 2 < e && w2 < v) +
+                   (int)(a + 3 < e && w3 < v);
+        for (int j = a + 4; j < e; j++) rank += ok[j] < v;
         R[start + out + a + rank] = v;
         C[start + out + a + rank] = (int32_t)oc[i];
       }
