@@ -270,10 +270,14 @@ __device__ __forceinline__ uint32_t match4(uint32_t w, uint32_t e) {
   return ((z >> 7) & 1u) | ((z >> 14) & 2u) | ((z >> 21) & 4u) | ((z >> 28) & 8u);
 }
 
-// Masks (bit i = byte base + i) of the event bytes (e0 or e1) and of the
-// '>' bytes among the 16 bytes at base (bytes at or past n never match).
-__device__ __forceinline__ void scan16(const uint8_t* fa, uint64_t n, uint64_t base, uint32_t e0,
-                                       uint32_t e1, uint32_t& ev, uint32_t& gt) {
+// Masks (bit i = byte base + i) of the event bytes (e0 or e1) and, when
+// want_gt, of the '>' bytes among the 16 bytes at base (bytes at or past n
+// never match); returns the 16 bytes (zero past n).  The kernels are bound
+// by these byte matches, so a second match runs only when e0 != e1 (the
+// builder's delimiter and skip symbol; the main path looks for '\n' alone)
+// and the '>' match only where a caller needs it.
+__device__ __forceinline__ uint4 scan16(const uint8_t* fa, uint64_t n, uint64_t base, uint32_t e0, uint32_t e1,
+                                        bool want_gt, uint32_t& ev, uint32_t& gt) {
   ev = 0;
   gt = 0;
   if (base + 16 <= n && ((reinterpret_cast<uintptr_t>(fa + base) & 15) == 0)) {
@@ -281,17 +285,22 @@ __device__ __forceinline__ void scan16(const uint8_t* fa, uint64_t n, uint64_t b
     const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      ev |= (match4(w[i], e0) | match4(w[i], e1)) << (4 * i);
-      gt |= match4(w[i], '>') << (4 * i);
+      uint32_t m = match4(w[i], e0);
+      if (e1 != e0) m |= match4(w[i], e1);
+      ev |= m << (4 * i);
+      if (want_gt) gt |= match4(w[i], '>') << (4 * i);
     }
-  } else {
-    for (int i = 0; i < 16; i++) {
-      if (base + i >= n) break;
-      const uint32_t b = fa[base + i];
-      ev |= (uint32_t)(b == e0 || b == e1) << i;
-      gt |= (uint32_t)(b == '>') << i;
-    }
+    return v;
   }
+  uint32_t w[4] = {0, 0, 0, 0};
+  for (int i = 0; i < 16; i++) {
+    if (base + i >= n) break;
+    const uint32_t b = fa[base + i];
+    w[i >> 2] |= b << (8 * (i & 3));
+    ev |= (uint32_t)(b == e0 || b == e1) << i;
+    gt |= (uint32_t)(b == '>') << i;
+  }
+  return make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 // scan16 that also returns the 16 bytes (zero past n).
@@ -322,13 +331,16 @@ __device__ __forceinline__ uint8_t byte_of(const uint4& v, int i) {
 
 // Event bytes of the index: bytes equal to e0 or e1 ('\n' and '\n' for the
 // line index of the main path; delimiter and skip symbol for the builder).
-// Per block: the number of event bytes and the first '>'.
+// Per block: the number of event bytes and the first '>'.  Only the input's
+// first '>' is wanted, so only block 0 looks for it unless gt_all is set
+// (the host asks for that pass when block 0 holds no '>').
 __global__ __launch_bounds__(NL_NT) void k_nl_count(const uint8_t* fa, uint64_t n, uint8_t e0,
-                                                    uint8_t e1, ull* blk_cnt, ull* blk_gt) {
+                                                    uint8_t e1, ull* blk_cnt, ull* blk_gt, int gt_all) {
   const uint64_t b0 = (uint64_t)blockIdx.x * NL_PER_BLOCK + (uint64_t)threadIdx.x * 16;
+  const bool want_gt = gt_all || blockIdx.x == 0;
   uint32_t ev[NL_SLABS], gm[NL_SLABS];
 #pragma unroll
-  for (int j = 0; j < NL_SLABS; j++) scan16(fa, n, b0 + j * NL_SLAB, e0, e1, ev[j], gm[j]);
+  for (int j = 0; j < NL_SLABS; j++) scan16(fa, n, b0 + j * NL_SLAB, e0, e1, want_gt, ev[j], gm[j]);
   ull c = 0, gt = ~0ull;
 #pragma unroll
   for (int j = NL_SLABS - 1; j >= 0; j--) {
@@ -363,11 +375,13 @@ constexpr ull NL_POS_MASK = (1ull << NL_FLAG_SHIFT) - 1;
 __global__ __launch_bounds__(NL_NT) void k_nl_write(const uint8_t* fa, uint64_t n, uint8_t e0,
                                                     uint8_t e1, const ull* blk_off, ull* nl, int follow) {
   const uint64_t b0 = (uint64_t)blockIdx.x * NL_PER_BLOCK + (uint64_t)threadIdx.x * 16;
-  uint32_t ev[NL_SLABS], gm[NL_SLABS];
+  uint32_t ev[NL_SLABS];
+  uint4 by[NL_SLABS];
   ull packed = 0;
 #pragma unroll
   for (int j = 0; j < NL_SLABS; j++) {
-    scan16(fa, n, b0 + j * NL_SLAB, e0, e1, ev[j], gm[j]);
+    uint32_t gm;
+    by[j] = scan16(fa, n, b0 + j * NL_SLAB, e0, e1, false, ev[j], gm);
     packed |= (ull)__popc(ev[j]) << (16 * j);
   }
   ull tot;
@@ -384,7 +398,7 @@ __global__ __launch_bounds__(NL_NT) void k_nl_write(const uint8_t* fa, uint64_t 
       if (follow) {
         // the next byte: in this thread's 16 bytes, else the next chunk's
         // first (the neighbour's load, usually still in cache)
-        f = p + 1 >= n ? 2ull : i < 15 ? (ull)((gm[j] >> (i + 1)) & 1u) : (ull)(fa[p + 1] == '>');
+        f = p + 1 >= n ? 2ull : i < 15 ? (ull)(byte_of(by[j], i + 1) == '>') : (ull)(fa[p + 1] == '>');
       }
       nl[o++] = p | (f << NL_FLAG_SHIFT);
     }
@@ -3142,7 +3156,7 @@ int event_index(pk_ctx* c, const uint8_t* fa, uint64_t n, uint8_t e0, uint8_t e1
   PK_TRY(need(c, B_BLKGT, nb + 1, &blkgt));
   PK_TRY(need(c, B_BLKOFF, nb + 1, &blkoff));
   if (nb) {
-    hipLaunchKernelGGL(k_nl_count, dim3((unsigned)nb), dim3(NL_NT), 0, st, fa, n, e0, e1, blk, blkgt);
+    hipLaunchKernelGGL(k_nl_count, dim3((unsigned)nb), dim3(NL_NT), 0, st, fa, n, e0, e1, blk, blkgt, 0);
     PK_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(1024), 0, st, blkgt, nb, small);
     PK_HIP(c, hipGetLastError());
@@ -3152,6 +3166,15 @@ int event_index(pk_ctx* c, const uint8_t* fa, uint64_t n, uint8_t e0, uint8_t e1
   PK_HIP(c, hipMemcpyAsync(&hm[0], blkoff + nb, 8, hipMemcpyDeviceToHost, st));
   PK_HIP(c, hipMemcpyAsync(&hm[1], small, 8, hipMemcpyDeviceToHost, st));
   PK_HIP(c, hipStreamSynchronize(st));
+  if (hm[1] == ~0ull && nb > 1) {
+    // no '>' in the first block: the first one anywhere (same counts again)
+    hipLaunchKernelGGL(k_nl_count, dim3((unsigned)nb), dim3(NL_NT), 0, st, fa, n, e0, e1, blk, blkgt, 1);
+    PK_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(1024), 0, st, blkgt, nb, small);
+    PK_HIP(c, hipGetLastError());
+    PK_HIP(c, hipMemcpyAsync(&hm[1], small, 8, hipMemcpyDeviceToHost, st));
+    PK_HIP(c, hipStreamSynchronize(st));
+  }
   const uint64_t m = hm[0];
   PK_TRY(need(c, ev_buf, m + 1, &nl));
   if (nb) {

@@ -116,6 +116,10 @@ def test_format_errors(ctx, oracle_mod):
     b"junk before\n>h\n" + b"ACGTTGCA" * 8 + b"\n",
     b">h\n" + b"ACGTNNacgtRYKM" * 9 + b"\n>h2\n" + b"nNnNACGT" * 7,
     b">h\r\n" + b"ACGT" * 20 + b"\n",
+    # no '>' in the line index's first 16 KB block: the first one is found
+    # by a second pass over every block (and none at all gives no records)
+    b"junk\n" * 5000 + b">h\n" + b"ACGTTGCA" * 8 + b"\n>h2\n" + b"TTGCAACG" * 9 + b"\n",
+    b"junk line without a record\n" * 2000,
 ])
 def test_edge_inputs(ctx, oracle_mod, fa):
     _p(f"case: edge input len={len(fa)}")
