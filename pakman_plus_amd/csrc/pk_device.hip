@@ -2166,13 +2166,32 @@ __global__ void k_compact(const Key* S, const int32_t* C, const ull* lstart, con
 
 constexpr int CUT_NT = 1024;
 constexpr int T_MAX = 4096;
+// A cutoff block covers CUT_R rounds of CUT_NT * 4 counts: in round r thread
+// t reads the four at r * CUT_NT * 4 + 4 t (one 16-byte load, coalesced),
+// and its loads of all rounds are in flight together.
+constexpr int CUT_R = 4;
+constexpr uint64_t CUT_PER = (uint64_t)CUT_R * CUT_NT * 4;
+
+__device__ __forceinline__ void cut_load4(const int32_t* TC, uint64_t U, uint64_t base, int32_t fill,
+                                          int32_t* v) {
+  if (base + 4 <= U) {
+    const int4 q = *reinterpret_cast<const int4*>(TC + base);
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+    v[3] = q.w;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; i++) v[i] = base + i < U ? TC[base + i] : fill;
+  }
+}
 
 __global__ __launch_bounds__(CUT_NT) void k_count_hist(const int32_t* TC, uint64_t U, int t, ull* hist) {
   __shared__ uint32_t h[T_MAX];
   for (int i = threadIdx.x; i < t; i += CUT_NT) h[i] = 0;
   __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER;
-  for (uint64_t i = base + threadIdx.x; i < U && i < base + SCAN_PER; i += CUT_NT) {
+  const uint64_t base = (uint64_t)blockIdx.x * CUT_PER;
+  for (uint64_t i = base + threadIdx.x; i < U && i < base + CUT_PER; i += CUT_NT) {
     const int v = TC[i];
     if (v >= 0 && v < t) atomicAdd(&h[v], 1u);
   }
@@ -2194,23 +2213,15 @@ __global__ __launch_bounds__(CUT_NT) void k_count_hist_blocks(const int32_t* TC,
   const int w = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < NW * t; i += CUT_NT) h[i] = 0;
   __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER + (uint64_t)threadIdx.x * 4;
-  int v[4];
-  if (base + 4 <= U) {
-    const int4 q = *reinterpret_cast<const int4*>(TC + base);
-    v[0] = q.x;
-    v[1] = q.y;
-    v[2] = q.z;
-    v[3] = q.w;
-  } else {
+  const uint64_t base = (uint64_t)blockIdx.x * CUT_PER + (uint64_t)threadIdx.x * 4;
+  int32_t v[CUT_R * 4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) v[i] = base + i < U ? TC[base + i] : -1;
-  }
+  for (int r = 0; r < CUT_R; r++) cut_load4(TC, U, base + (uint64_t)r * CUT_NT * 4, -1, v + 4 * r);
   // count 1 (the k-mers seen once, most of the table) is summed per wave by
   // ballot instead of 64 lanes hitting one LDS word
   const int lane = threadIdx.x & 63;
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
+  for (int i = 0; i < CUT_R * 4; i++) {
     const bool one = v[i] == 1 && t > 1;
     const unsigned long long m = __ballot(one);
     if (lane == 0 && m) atomicAdd(&h[w * t + 1], (uint32_t)__popcll(m));
@@ -2240,7 +2251,7 @@ __global__ void k_kept_from_hist(const uint32_t* BH, uint64_t nb, uint64_t U, in
   const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   const int m = *mb;
-  const uint64_t size = (b + 1) * SCAN_PER <= U ? SCAN_PER : U - b * SCAN_PER;
+  const uint64_t size = (b + 1) * CUT_PER <= U ? CUT_PER : U - b * CUT_PER;
   uint64_t below = 0;
   for (int c = 0; c < m && c < t; c++) below += BH[b * t + c];
   kept[b] = size - below;
@@ -2261,11 +2272,13 @@ __global__ void k_min_bucket(const ull* hist, int t, int* mb) {
 __global__ __launch_bounds__(CUT_NT) void k_kept_count(const int32_t* TC, uint64_t U, const int* mb,
                                                       ull* bsum) {
   const int thr = *mb;
-  const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER + (uint64_t)threadIdx.x * 4;
+  const uint64_t base = (uint64_t)blockIdx.x * CUT_PER + (uint64_t)threadIdx.x * 4;
+  int32_t v[CUT_R * 4];
+#pragma unroll
+  for (int r = 0; r < CUT_R; r++) cut_load4(TC, U, base + (uint64_t)r * CUT_NT * 4, INT32_MIN, v + 4 * r);
   ull c = 0;
 #pragma unroll
-  for (int i = 0; i < 4; i++)
-    if (base + i < U) c += TC[base + i] >= thr;
+  for (int i = 0; i < CUT_R * 4; i++) c += v[i] >= thr;
   c = block_sum<CUT_NT>(c);
   if (threadIdx.x == 0) bsum[blockIdx.x] = c;
 }
@@ -2278,29 +2291,37 @@ __global__ __launch_bounds__(CUT_NT) void k_kept_write(const Key* TK, const int3
   // thousand of a billion k-mers); the test is the same for every thread
   if (boffs[blockIdx.x + 1] == boffs[blockIdx.x]) return;
   const int thr = *mb;
-  const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER + (uint64_t)threadIdx.x * 4;
-  int32_t cv[4];
-  if (base + 4 <= U) {
-    const int4 q = *reinterpret_cast<const int4*>(TC + base);
-    cv[0] = q.x;
-    cv[1] = q.y;
-    cv[2] = q.z;
-    cv[3] = q.w;
-  } else {
+  const uint64_t base = (uint64_t)blockIdx.x * CUT_PER + (uint64_t)threadIdx.x * 4;
+  int32_t cv[CUT_R * 4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) cv[i] = base + i < U ? TC[base + i] : 0;
+  for (int r = 0; r < CUT_R; r++) cut_load4(TC, U, base + (uint64_t)r * CUT_NT * 4, INT32_MIN, cv + 4 * r);
+  // the block's kept entries go out in index order: round by round, and in
+  // a round by thread.  A thread's four round counts (<= 4 each, <= 4096
+  // per round over the block) travel through one scan in 16-bit fields.
+  ull packed = 0;
+#pragma unroll
+  for (int r = 0; r < CUT_R; r++) {
+    ull c = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) c += cv[4 * r + i] >= thr;
+    packed |= c << (16 * r);
   }
-  ull c = 0;
+  ull tot;
+  const ull ex = block_scan_total<CUT_NT>(packed, &tot);
+  ull rbase = boffs[blockIdx.x];
 #pragma unroll
-  for (int i = 0; i < 4; i++) c += (base + i < U) && cv[i] >= thr;
-  ull o = block_excl_scan<CUT_NT>(c) + boffs[blockIdx.x];
+  for (int r = 0; r < CUT_R; r++) {
+    ull o = rbase + ((ex >> (16 * r)) & 0xFFFFull);
+    const uint64_t b = base + (uint64_t)r * CUT_NT * 4;
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    if (base + i < U && cv[i] >= thr) {
-      KK[o] = TK[base + i];
-      KC[o] = cv[i];
-      o++;
+    for (int i = 0; i < 4; i++) {
+      if (cv[4 * r + i] >= thr) {
+        KK[o] = TK[b + i];
+        KC[o] = cv[4 * r + i];
+        o++;
+      }
     }
+    rbase += (tot >> (16 * r)) & 0xFFFFull;
   }
 }
 
@@ -3816,7 +3837,7 @@ int cutoff_impl(pk_ctx* c, int t, uint64_t* hist_out, int* mb_out, uint64_t* nk_
   PK_TRY(need(c, B_HIST, (uint64_t)t + 8, &hist));
   PK_TRY(need(c, B_SMALL, 16, &small));
   PK_TRY(fill_u64(c, hist, 0, (uint64_t)t));
-  const uint64_t nb = (U + SCAN_PER - 1) / SCAN_PER;
+  const uint64_t nb = (U + CUT_PER - 1) / CUT_PER;
   int* mbd = reinterpret_cast<int*>(small + 8);
   ull *bs, *boffs;
   PK_TRY(need(c, B_KBS, nb + 1, &bs));
