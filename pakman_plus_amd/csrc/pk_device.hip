@@ -213,19 +213,44 @@ __global__ void k_diff(const ull* off, uint64_t n, ull* out) {
 }
 
 // ---------------------------------------------------------------------------
-// Exclusive scan (u32 or u64 in, u64 out, out[n] = total).  4096 elements
-// per 1024-thread block, recursive over block sums.
+// Exclusive scan (u32 or u64 in, u64 out, out[n] = total).  SCAN_PER
+// elements per 1024-thread block, SCAN_EPT consecutive ones per thread
+// (read and written as 16-byte vectors where aligned, so a thread's loads
+// are in flight together), recursive over block sums.
 
 constexpr int SCAN_NT = 1024;
-constexpr int SCAN_PER = 4096;
+constexpr int SCAN_EPT = 16;
+constexpr int SCAN_PER = SCAN_NT * SCAN_EPT;
+
+// elements [base, base + SCAN_EPT) of in, zero past n
+template <class T>
+__device__ __forceinline__ void scan_load(const T* in, uint64_t n, uint64_t base, ull* v) {
+  if (base + SCAN_EPT <= n && (reinterpret_cast<uintptr_t>(in + base) & 15) == 0) {
+    constexpr int PV = 16 / (int)sizeof(T);  // elements per 16-byte vector
+    const uint4* p = reinterpret_cast<const uint4*>(in + base);
+    uint4 q[SCAN_EPT / PV];
+#pragma unroll
+    for (int j = 0; j < SCAN_EPT / PV; j++) q[j] = p[j];
+#pragma unroll
+    for (int j = 0; j < SCAN_EPT / PV; j++) {
+      const T* e = reinterpret_cast<const T*>(&q[j]);
+#pragma unroll
+      for (int i = 0; i < PV; i++) v[j * PV + i] = (ull)e[i];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < SCAN_EPT; i++) v[i] = base + i < n ? (ull)in[base + i] : 0ull;
+  }
+}
 
 template <class T>
 __global__ __launch_bounds__(SCAN_NT) void k_scan_reduce(const T* in, ull* sums, uint64_t n) {
-  const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER + (uint64_t)threadIdx.x * 4;
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER + (uint64_t)threadIdx.x * SCAN_EPT;
+  ull v[SCAN_EPT];
+  scan_load<T>(in, n, base, v);
   ull s = 0;
 #pragma unroll
-  for (int i = 0; i < 4; i++)
-    if (base + i < n) s += (ull)in[base + i];
+  for (int i = 0; i < SCAN_EPT; i++) s += v[i];
   s = block_sum<SCAN_NT>(s);
   if (threadIdx.x == 0) sums[blockIdx.x] = s;
 }
@@ -233,23 +258,32 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_reduce(const T* in, ull* sums,
 template <class T>
 __global__ __launch_bounds__(SCAN_NT) void k_scan_apply(const T* in, ull* out, const ull* offs,
                                                         uint64_t n) {
-  const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER + (uint64_t)threadIdx.x * 4;
-  ull v[4];
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER + (uint64_t)threadIdx.x * SCAN_EPT;
+  ull v[SCAN_EPT];
+  scan_load<T>(in, n, base, v);
   ull s = 0;
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    v[i] = (base + i < n) ? (ull)in[base + i] : 0ull;
-    s += v[i];
-  }
-  ull ex = block_excl_scan<SCAN_NT>(s);
+  for (int i = 0; i < SCAN_EPT; i++) s += v[i];
+  const ull ex = block_excl_scan<SCAN_NT>(s);
   ull o = (offs ? offs[blockIdx.x] : 0ull) + ex;
+  if (base + SCAN_EPT <= n && (reinterpret_cast<uintptr_t>(out + base) & 15) == 0) {
+    ulonglong2* q = reinterpret_cast<ulonglong2*>(out + base);
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    if (base + i < n) out[base + i] = o;
-    o += v[i];
+    for (int j = 0; j < SCAN_EPT / 2; j++) {
+      const ull a = o;
+      o += v[2 * j];
+      q[j] = make_ulonglong2(a, o);
+      o += v[2 * j + 1];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < SCAN_EPT; i++) {
+      if (base + i < n) out[base + i] = o;
+      o += v[i];
+    }
   }
   // the thread owning element n-1 writes the total
-  if (n > 0 && base <= n - 1 && n - 1 < base + 4) out[n] = o;
+  if (n > 0 && base <= n - 1 && n - 1 < base + SCAN_EPT) out[n] = o;
 }
 
 // ---------------------------------------------------------------------------
@@ -3167,7 +3201,8 @@ int resolve_overflow(pk_ctx* c, Key* S, Key* R, int32_t* C, int K, int fixed_bit
 }
 
 // Index of the event bytes ('\n', or '\n' and '>') of fa: *m events at
-// *ev_out.  small[0] receives the position of the first '>'.
+// *ev_out; *p0_out = the position of the first '>' (n when there is none).
+// small[0] is scratch for the pass that looks for it in every block.
 int event_index(pk_ctx* c, const uint8_t* fa, uint64_t n, uint8_t e0, uint8_t e1, ull* small,
                 ull** ev_out, uint64_t* m_out, uint64_t* p0_out, int ev_buf = B_NL, int follow = 0) {
   hipStream_t st = c->stream;
@@ -3179,13 +3214,13 @@ int event_index(pk_ctx* c, const uint8_t* fa, uint64_t n, uint8_t e0, uint8_t e1
   if (nb) {
     hipLaunchKernelGGL(k_nl_count, dim3((unsigned)nb), dim3(NL_NT), 0, st, fa, n, e0, e1, blk, blkgt, 0);
     PK_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(1024), 0, st, blkgt, nb, small);
-    PK_HIP(c, hipGetLastError());
   }
   PK_TRY(scan_excl<ull>(c, blk, blkoff, nb));
-  ull hm[2];
+  // a '>' in the first block is the input's first one (other blocks report
+  // one only from their byte-wise tail, which comes after it)
+  ull hm[2] = {0, ~0ull};
   PK_HIP(c, hipMemcpyAsync(&hm[0], blkoff + nb, 8, hipMemcpyDeviceToHost, st));
-  PK_HIP(c, hipMemcpyAsync(&hm[1], small, 8, hipMemcpyDeviceToHost, st));
+  if (nb) PK_HIP(c, hipMemcpyAsync(&hm[1], blkgt, 8, hipMemcpyDeviceToHost, st));
   PK_HIP(c, hipStreamSynchronize(st));
   if (hm[1] == ~0ull && nb > 1) {
     // no '>' in the first block: the first one anywhere (same counts again)
