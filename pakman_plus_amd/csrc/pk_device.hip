@@ -1224,16 +1224,14 @@ __global__ __launch_bounds__((L1Scatter<Key, ENC, OWN>::NT)) void k_l1_scatter(c
   constexpr int NT = L1Scatter<Key, ENC, OWN>::NT;
   constexpr int GF = L1Scatter<Key, ENC, OWN>::GF;
   __shared__ ChunkLds C;
-  // dynamic LDS: plane A, 2^d1 output cursors, plane B, 2^d1 ranks
+  // dynamic LDS: plane A, 2^d1 output cursors, plane B (then 2^d1 words
+  // the cursors no longer need: each k-mer takes its slot from one 64-bit
+  // atomic on its digit's cursor)
   extern __shared__ ull l1_dyn[];
   const int nb = 1 << d1;
-  ull* base = l1_dyn + sb / 32;
-  const Stage sg{l1_dyn, reinterpret_cast<uint32_t*>(base + nb), sb};
-  uint32_t* h = sg.B + sb / 32;
-  for (int i = threadIdx.x; i < nb; i += NT) {
-    base[i] = O[(ull)i * nblk_h + (ull)blockIdx.x * GF];
-    h[i] = 0;
-  }
+  ull* cur = l1_dyn + sb / 32;
+  const Stage sg{l1_dyn, reinterpret_cast<uint32_t*>(cur + nb), sb};
+  for (int i = threadIdx.x; i < nb; i += NT) cur[i] = O[(ull)i * nblk_h + (ull)blockIdx.x * GF];
   const uint64_t r0 = (uint64_t)blockIdx.x * GF * rpb;
   const uint64_t r1 = r0 + (uint64_t)GF * rpb < R ? r0 + (uint64_t)GF * rpb : R;
   const int bpos = KB - d1;
@@ -1249,7 +1247,7 @@ __global__ __launch_bounds__((L1Scatter<Key, ENC, OWN>::NT)) void k_l1_scatter(c
       walk_read<Key, ENC, OWN>(C, sg, fa, rstart[r], rlen[r], i, km, os, cf, K, mask, 0x7fffffff,
                                [&](int, Key x) {
                                  const uint32_t d = key_digit<Key>(x, bpos, d1);
-                                 S1[base[d] + atomicAdd(&h[d], 1u)] = x;
+                                 S1[atomicAdd(&cur[d], 1ull)] = x;
                                });
     }
     c0 += m;
